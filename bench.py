@@ -1,0 +1,212 @@
+"""Benchmark of the near-Toeplitz hot path on one MI355X.
+
+Headline workload: 4th-order compact d/dx of a 512^3 FP64 field
+f = sin(2 pi x) sin(2 pi y) sin(2 pi z) + N(0, 1e-3) noise (seed 0), i.e.
+262,144 near-Toeplitz systems of size 512, one fused launch per step (RHS
+stencil + cyclic reduction, each point read once and written once).
+Metric: grid points per second and the fraction of HBM peak.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick]
+
+Prints ONE JSON line.  Other configurations (d/dy, d/dz, FP32 + refinement,
+FP32 storage at 1024^3, ADI 2048^2 x 200 steps in a hipGraph) are timed after
+the headline and reported under "extras".
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "near-toeplitz-applications_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_launches(fn, steps, warmup):
+    """Warm up, then time `steps` launches: per-launch HIP events on the current
+    stream (the stream the kernels go to) plus a wall clock around all of them."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [a.elapsed_time(b) * 1e-3 for a, b in evs]
+    return {"wall_s": wall, "mean_s": sum(per) / len(per), "median_s": statistics.median(per),
+            "min_s": min(per), "max_s": max(per)}
+
+
+def compact_case(N, axis, precision, sweeps, in_dtype, steps, warmup, field=None):
+    import torch
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.workloads import compact_field_device
+    f = field if field is not None else compact_field_device(N, seed=0, dtype=in_dtype)
+    out = torch.empty(f.shape, dtype=torch.float64, device=f.device)
+    plan = Plan(N, COMPACT_COEFFS)
+    h = 1.0 / (N - 1)
+    t = time_launches(lambda: core.compact_derivative(plan, f, out, h, axis, precision, sweeps),
+                      steps, warmup)
+    pts = N ** 3
+    nbytes = pts * (f.element_size() + 8)
+    t["pts_per_s"] = pts / t["mean_s"]
+    t["alg_bytes"] = nbytes
+    t["gbps"] = nbytes / t["mean_s"] / 1e9
+    t["hbm_frac"] = nbytes / t["mean_s"] / HBM_PEAK
+    return t, f, out
+
+
+def cpu_baseline(N, sample_lines, reps=3):
+    """SciPy banded solve (LAPACK gtsv) + NumPy RHS on a sample of the 512^3
+    d/dx workload's lines: the CPU path of the same matrices."""
+    import numpy as np
+    from oracle.compact import COMPACT_COEFFS, compact_rhs
+    from oracle.cr import banded_solve
+    rng = np.random.default_rng(0)
+    x = np.linspace(0.0, 1.0, N)
+    ys = rng.random(sample_lines)
+    zs = rng.random(sample_lines)
+    f = (np.sin(2 * np.pi * zs)[:, None] * np.sin(2 * np.pi * ys)[:, None]
+         * np.sin(2 * np.pi * x)[None, :]) + rng.normal(0.0, 1e-3, (sample_lines, N))
+    h = 1.0 / (N - 1)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        banded_solve(COMPACT_COEFFS, compact_rhs(f, h))
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    try:
+        from threadpoolctl import threadpool_info
+        blas_threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:
+        blas_threads = None
+    return {"value": sample_lines * N / med, "unit": "grid pts/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{sample_lines} random x-lines of the 512^3 d/dx workload "
+                      f"({sample_lines * N} pts), NumPy RHS + one scipy.linalg.solve_banded "
+                      f"call (LAPACK gtsv, single-threaded), median of {reps}",
+            "seconds": med, "host_cores_available": len(os.sched_getaffinity(0)),
+            "blas_threads": blas_threads}
+
+
+def adi_case(N, r, nsteps, reps):
+    import torch
+    from adi.solver import ADISolver
+    from ntamd.workloads import gaussian_ic
+    u = torch.tensor(gaussian_ic(N, seed=0), device="cuda")
+    s = ADISolver(N, r)
+    s.capture(u, nsteps)
+    s.replay()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        s.replay()
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b) * 1e-3)
+    t = statistics.median(ts)
+    half_bytes = 16 * N * N
+    return {"N": N, "r": r, "steps": nsteps, "s_per_run": t, "ms_per_step": 1e3 * t / nsteps,
+            "effective_gbps": 2 * nsteps * half_bytes / t / 1e9,
+            "note": "32 MiB field stays in the 256 MB Infinity Cache: effective, not HBM, GB/s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--quick", action="store_true", help="headline only, no extras/CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(lines shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ntamd import _lib
+    _lib.load()
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+
+    N = 512
+    log(f"[bench] headline: compact d/dx {N}^3 FP64, steps={args.steps} warmup={args.warmup}")
+    head, f64, out = compact_case(N, 2, "fp64", 1, torch.float64, args.steps, args.warmup)
+    log(f"[bench] d/dx: {head['mean_s'] * 1e6:.1f} us/launch, {head['gbps']:.0f} GB/s")
+
+    extras = {}
+    if not args.quick:
+        short = max(10, args.steps // 2)
+        for name, axis in (("ddy_fp64", 1), ("ddz_fp64", 0)):
+            t, _, _ = compact_case(N, axis, "fp64", 1, torch.float64, short, 3, field=f64)
+            extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
+            log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, {t['gbps']:.0f} GB/s")
+        for name, sweeps in (("ddx_fp32_refine1", 1), ("ddx_fp32_refine2", 2)):
+            t, _, o2 = compact_case(N, 2, "fp32_refine", sweeps, torch.float64, short, 3, field=f64)
+            err = ((o2 - out).abs().max() / out.abs().max()).item()
+            extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
+            extras[name]["max_rel_err_vs_fp64"] = err
+            log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, err {err:.2e}")
+        del out, f64
+        torch.cuda.empty_cache()
+        a = adi_case(2048, 0.5, 200, 3)
+        extras["adi_2048_200steps_graph"] = a
+        log(f"[bench] ADI 2048^2: {a['ms_per_step']:.3f} ms/step")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (SciPy banded) ...")
+        cpu = cpu_baseline(N, 131072)
+        log(f"[bench] CPU: {cpu['value']:.3e} pts/s")
+
+    pts = N ** 3
+    rec = {
+        "metric": "grid pts/s, 512^3 compact d/dx (FP64)",
+        "value": pts / head["mean_s"],
+        "unit": "grid pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["mean_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (sin(2pi x)sin(2pi y)sin(2pi z) + N(0,1e-3), torch seed 0, on device)",
+        "config": {"workload": "compact 4th-order d/dx, 512^3 FP64, 262144 systems x 512",
+                   "n": N, "systems": N * N, "coeffs": [1, 2, 0.25, 1, 0.25, 2, 1],
+                   "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": head["gbps"], "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": head["hbm_frac"], "traffic": None,
+                     "alg_bytes_per_launch": head["alg_bytes"],
+                     "kernel_mean_s": head["mean_s"], "kernel_median_s": head["median_s"]},
+        "cpu_baseline": cpu,
+        "wall_s_timed_region": head["wall_s"],
+        "extras": extras,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

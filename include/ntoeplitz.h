@@ -1,0 +1,118 @@
+/*
+ * ntoeplitz.h - C ABI of the MI355X near-Toeplitz tridiagonal library
+ * (libntoeplitz.so, HIP kernels for gfx950).
+ *
+ * Device pointers are plain HIP device addresses (e.g. torch tensor
+ * data_ptr()), streams are hipStream_t passed as void* (NULL = the null
+ * stream).  Every entry point returns NT_OK (0) or a negative NT_E* code and
+ * then leaves a message for nt_last_error() (per host thread).  No entry point
+ * allocates device memory on a solve path; launches are asynchronous on the
+ * given stream, as the reference's PyCUDA launches are on the default stream.
+ *
+ * The reference (ashwinsrnth/near-toeplitz-applications) is Python 2 +
+ * PyCUDA; each entry below names the reference interface it replaces.
+ */
+#ifndef NTOEPLITZ_H
+#define NTOEPLITZ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NT_OK 0
+#define NT_EINVAL (-1)      /* bad argument: size, shape, pointer, stride */
+#define NT_ESINGULAR (-2)   /* zero pivot in the cyclic reduction */
+#define NT_EHIP (-3)        /* HIP runtime error (launch, graph, device) */
+#define NT_EUNSUPPORTED (-4)
+
+/* element types of device buffers */
+#define NT_F64 0
+#define NT_F32 1
+
+/* arithmetic of the bulk solve */
+#define NT_PREC_F64 0         /* FP64 cyclic reduction */
+#define NT_PREC_F32_REFINE 1  /* FP32 cyclic reduction + FP64-residual refinement sweeps */
+
+typedef struct nt_plan nt_plan;
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* nt_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int nt_version(void);
+
+/* ---- plans ---------------------------------------------------------------
+ * Replaces NearToeplitzSolver.__init__(n, nrhs, coeffs) (neato.py:131-166):
+ * coeffs = (b1, c1, ai, bi, ci, an, bn) as in neato.py:136-138.  The level
+ * coefficients (neato.py:199-274, here for any 2 <= n <= 4096) are computed
+ * once on the host in FP64; the plan is immutable and usable from any stream.
+ * No device memory is held: kernels receive the table as a launch argument. */
+int nt_plan_create(int64_t n, const double coeffs[7], nt_plan** out);
+int nt_plan_destroy(nt_plan* plan);
+int nt_plan_info(const nt_plan* plan, int64_t* n, int* nlevels);
+/* Level table: out[8*l + {0..7}] = (m, bF, cF, a, b, c, aL, bL) of level l,
+ * l = 0 .. nlevels-1.  cap = number of doubles available in out. */
+int nt_plan_levels(const nt_plan* plan, double* out, int cap);
+/* The reference's eight arrays (a, b, c, k1, k2, b_first, k1_first, k1_last),
+ * each of length log2(n), concatenated in that order; power-of-2 n >= 4 only
+ * (neato.py:199-274). */
+int nt_plan_reference_arrays(const nt_plan* plan, double* out, int cap);
+
+/* ---- solves ---------------------------------------------------------------
+ * Replaces NearToeplitzSolver.solve(x_d) (neato.py:168-197): solves nrhs
+ * systems of length n stored contiguously from x (x may point into a larger
+ * array, as run-adi.py:65 passes d_x_gpu[1,0]), in place.
+ * dtype NT_F64 with precision NT_PREC_F64 or NT_PREC_F32_REFINE (sweeps >= 0). */
+int nt_solve(const nt_plan* plan, void* x, int dtype, int64_t nrhs, int precision, int sweeps,
+             void* stream);
+
+/* In-place solve of every line along `axis` (0 = z, 1 = y, 2 = x) of a
+ * C-contiguous (nz, ny, nx) field: the reference's permute + solve + permute
+ * (run-compact.py:66-90) without the permutes. */
+int nt_solve_axis(const nt_plan* plan, void* x, int dtype, int64_t nz, int64_t ny, int64_t nx,
+                  int axis, int precision, int sweeps, void* stream);
+
+/* ---- compact derivative (fused) ------------------------------------------
+ * out = d f / d axis by the 4th-order compact scheme: RHS stencil
+ * (compact/rhs.py:21-36) + solve (neato.py) + the permutes of
+ * run-compact.py:66-90, in one pass over HBM.  The plan must be the compact
+ * matrix of size n = extent of `axis`; dx is the grid spacing.
+ * f_dtype NT_F64 or NT_F32 (FP32 storage), out_dtype NT_F64. */
+int nt_compact_derivative(const nt_plan* plan, const void* f, int f_dtype, void* out,
+                          int out_dtype, int64_t nz, int64_t ny, int64_t nx, int axis,
+                          double dx, int precision, int sweeps, void* stream);
+
+/* compact.rhs.compute_rhs(f, rhs, dx) (compact/rhs.py:47-53): RHS along x of
+ * a (nz, ny, nx) FP64 field. */
+int nt_compact_rhs(const void* f, void* rhs, int64_t nz, int64_t ny, int64_t nx, double dx,
+                   void* stream);
+
+/* compact.permute.permute(a_d, b_d, permutation) (compact/permute.py:32-52):
+ * b = a.transpose(perm) for a C-contiguous 3-D FP64 array of shape `shape`
+ * (any shape, not only cubes). */
+int nt_permute(const void* a, void* b, const int64_t shape[3], const int perm[3], void* stream);
+
+/* ---- ADI ----------------------------------------------------------------
+ * adi.rhs.compute_rhs(u, d, dx, dt, N) (adi/rhs.py:35-55): interior-only
+ * explicit stencil along the slow axis of an N x N FP64 array. */
+int nt_adi_rhs(const void* u, void* d, double dx, double dt, int64_t N, void* stream);
+
+/* adi.transpose.transpose(a, b, N) (adi/transpose.py:40-50): b = a^T, N x N FP64. */
+int nt_transpose(const void* a, void* b, int64_t N, void* stream);
+
+/* Fused Peaceman-Rachford steps (run-adi.py:60-71 without the transposes):
+ * each step is an x-implicit half step u -> work and a y-implicit half step
+ * work -> u, lines (-r/2, 1+r, -r/2) with identity (Dirichlet) ends.  `plan`
+ * must be built from (1, 0, -r/2, 1+r, -r/2, 0, 1) with n = N.  `work` is an
+ * N x N FP64 buffer whose boundary rows/columns must equal u's (Dirichlet
+ * values); nt_adi_prepare fills it. */
+int nt_adi_prepare(const void* u, void* work, int64_t N, void* stream);
+int nt_adi_steps(const nt_plan* plan, void* u, void* work, int64_t N, double r, int nsteps,
+                 int precision, int sweeps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NTOEPLITZ_H */

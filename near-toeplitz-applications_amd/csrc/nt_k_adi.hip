@@ -1,0 +1,10 @@
+// Line-kernel instantiations: fused Peaceman-Rachford half steps
+// (adi/rhs.py stencil + neato.py solve, no adi/transpose.py pass).
+#include "nt_launch.h"
+
+namespace nt {
+NT_INSTANTIATE(double, double, CONTIG, P_ADI, PREC_F64)
+NT_INSTANTIATE(double, double, STRIDED, P_ADI, PREC_F64)
+NT_INSTANTIATE(double, double, CONTIG, P_ADI, PREC_F32R)
+NT_INSTANTIATE(double, double, STRIDED, P_ADI, PREC_F32R)
+}  // namespace nt

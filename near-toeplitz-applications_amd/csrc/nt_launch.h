@@ -1,0 +1,60 @@
+// Launch-shape selection for the line kernel.  Each (input type, output type,
+// layout, producer, precision) combination is instantiated in its own
+// translation unit so the library builds in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nt_line_kernel.h"
+
+namespace nt {
+
+// Launch the line kernel over `outer` slabs of `a.phys` staged lines each.
+// Fills a.tiles.  Returns hipErrorInvalidValue for a size the kernel family
+// does not cover.
+template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
+hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
+
+#define NT_GO(R_, W_, WAVES_, E_)                                                          \
+  do {                                                                                     \
+    constexpr int TS_ = (WAVES_) * (64 / (W_));                                            \
+    const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
+    a.tiles = (nsys_ + TS_ - 1) / TS_;                                                     \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    const int64_t grid_ = outer * a.tiles;                                                 \
+    if (grid_ <= 0) return hipSuccess;                                                     \
+    if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
+    hipLaunchKernelGGL((line_kernel<TIn, TOut, R_, W_, WAVES_, E_, LAYOUT, PROD, PREC>),   \
+                       dim3(static_cast<unsigned>(grid_)), dim3((WAVES_) * 64), 0, st, a); \
+    return hipGetLastError();                                                              \
+  } while (0)
+
+template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
+hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
+  const int n = a.plan.n;
+  if (LAYOUT == CONTIG) {
+    // one system per 64-lane wavefront (16 lanes for tiny systems), 4 waves
+    if (n <= 16) NT_GO(1, 16, 4, 16);
+    if (n <= 64) NT_GO(1, 64, 4, 64);
+    if (n <= 128) NT_GO(2, 64, 4, 128);
+    if (n <= 256) NT_GO(4, 64, 4, 256);
+    if (n <= 512) NT_GO(8, 64, 4, 512);
+    if (n <= 1024) NT_GO(16, 64, 4, 1024);
+    if (n <= 2048) NT_GO(32, 64, 4, 2048);
+  } else {
+    // 16 adjacent lines per tile (128-byte rows in FP64), streamed in blocks
+    if (n <= 16) NT_GO(1, 16, 4, 16);
+    if (n <= 32) NT_GO(2, 16, 4, 32);
+    if (n <= 64) NT_GO(4, 16, 4, 64);
+    if (n <= 128) NT_GO(8, 16, 4, 128);
+    if (n <= 256) NT_GO(16, 16, 4, 128);
+    if (n <= 512) NT_GO(32, 16, 4, 128);
+    if (n <= 1024) NT_GO(16, 64, 16, 128);
+    if (n <= 2048) NT_GO(32, 64, 16, 128);
+  }
+  return hipErrorInvalidValue;
+}
+
+#define NT_INSTANTIATE(TIn, TOut, LAYOUT, PROD, PREC) \
+  template hipError_t launch_line<TIn, TOut, LAYOUT, PROD, PREC>(LineArgs&, int64_t, hipStream_t);
+
+}  // namespace nt

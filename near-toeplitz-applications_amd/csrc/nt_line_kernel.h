@@ -1,0 +1,243 @@
+// The line kernel: every batched near-Toeplitz solve in the library is one
+// launch of this template.
+//
+// A workgroup owns a tile of TS systems (lines) that are adjacent in memory:
+//   CONTIG   lines are contiguous rows (x-lines): system p starts at p*pstride,
+//            elements are unit-stride;
+//   STRIDED  lines are strided (y- or z-lines): TS consecutive "inner" indices
+//            (unit stride between systems), elements at estride.
+// The line is streamed through LDS in blocks of E elements for all TS (+ halo)
+// systems of the tile with fully coalesced global accesses, and each lane
+// copies its own chunk of R consecutive rows into registers (so the
+// permute/transpose of the reference, compact/permute.py and adi/transpose.py,
+// never touches HBM).  The producer forms the right-hand side on the fly
+// (reference compact/rhs.py:21-36, adi/rhs.py:21-24), the solve runs in
+// registers (nt_cr.h), and the solution goes back through LDS to HBM with the
+// same coalesced pattern: each field element is read once and written once.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nt_cr.h"
+#include "nt_plan.h"
+
+namespace nt {
+
+enum Layout : int { CONTIG = 0, STRIDED = 1 };
+enum Producer : int { P_IDENT = 0, P_COMPACT = 1, P_ADI = 2 };
+enum Precision : int { PREC_F64 = 0, PREC_F32R = 1 };
+
+struct LineArgs {
+  NtPlanK plan;
+  const void* in;
+  void* out;
+  int64_t pstride;     // distance between adjacent systems (CONTIG: line length)
+  int64_t estride;     // distance between adjacent elements (CONTIG: 1)
+  int64_t ostride;     // distance between outer slabs
+  int64_t phys;        // lines per slab including halo lines (ADI boundary lines)
+  int64_t tiles;       // tiles per slab
+  double s1, s2;       // producer scalars: compact 3/(4h), 1/(2h); ADI r/2, 1-r
+  int32_t sweeps;      // refinement sweeps (PREC_F32R)
+  int32_t pad_;
+};
+
+template <typename T>
+__device__ __forceinline__ double ld_as_double(const T* p) {
+  return static_cast<double>(*p);
+}
+
+template <typename T>
+__device__ __forceinline__ void st_from_double(T* p, double v) {
+  *p = static_cast<T>(v);
+}
+
+// residual r = d - A x for the level-0 near-Toeplitz matrix, chunked layout
+template <int R, int W>
+__device__ __forceinline__ void residual(const double (&d)[R], const double (&x)[R], double (&r)[R],
+                                         const NtPlanK& P, int lane) {
+  const double xl = shup<W>(x[R - 1], 1);
+  const double xr = shdn<W>(x[0], 1);
+  const int n = P.n;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int e = lane * R + k;
+    const double left = (k > 0) ? x[k - 1] : xl;
+    const double right = (k + 1 < R) ? x[k + 1] : xr;
+    double ax;
+    if (e == 0) {
+      ax = P.b1 * x[k] + P.c1 * right;
+    } else if (e == n - 1) {
+      ax = P.an * left + P.bn * x[k];
+    } else {
+      ax = P.ai * left + P.bi * x[k] + P.ci * right;
+    }
+    r[k] = (e < n) ? d[k] - ax : 0.0;
+  }
+}
+
+template <typename TIn, typename TOut, int R, int W, int WAVES, int E, int LAYOUT, int PROD, int PREC>
+__global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args) {
+  constexpr int NT = WAVES * 64;
+  constexpr int SPW = 64 / W;                 // systems (segments) per wave
+  constexpr int TS = WAVES * SPW;             // systems per tile
+  constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
+  constexpr int ROWS = TS + 2 * HALO;         // lines staged per block
+  constexpr int CP = (R >= 4) ? 2 : 0;        // pad doubles after each R-chunk (LDS banks)
+  constexpr int EP = (E / R) * (R + CP) + 2;  // LDS stride between staged lines
+  static_assert(E % R == 0, "a chunk must not straddle blocks");
+
+  __shared__ double lds[ROWS * EP];
+
+  const NtPlanK& P = args.plan;
+  const int n = P.n;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int seg = (tid & 63) / W;
+  const int lane = tid & (W - 1);
+  const int ql = wave * SPW + seg;            // tile-local system
+
+  const int64_t bid = blockIdx.x;
+  const int64_t o = bid / args.tiles;
+  const int64_t p0 = (bid - o * args.tiles) * TS;   // first staged line of the tile
+  const int64_t nsys = args.phys - 2 * HALO;
+  const bool active = (p0 + ql) < nsys;
+  const int64_t base = o * args.ostride + p0 * args.pstride;
+  const TIn* __restrict__ in = static_cast<const TIn*>(args.in) + base;
+  TOut* __restrict__ out = static_cast<TOut*>(args.out) + base;
+  const int64_t rows_valid64 = args.phys - p0;
+  const int rows_valid = rows_valid64 < ROWS ? static_cast<int>(rows_valid64) : ROWS;
+
+  const int my_e0 = lane * R;                 // first row of my chunk
+  const int my_blk = my_e0 / E;
+  const int nblk = (n + E - 1) / E;
+
+  double f[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k] = 0.0;
+  // compact closures need f[0..2] and f[n-1], f[n-2], f[n-3] of the line
+  double bnd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+
+  auto lds_at = [&](int row, int el) -> int { return row * EP + el + (el / R) * CP; };
+
+  // ---------------- load: blocks of E elements of all staged lines ----------
+  for (int b = 0; b < nblk; ++b) {
+    const int e0 = b * E;
+    if (b > 0) __syncthreads();
+    if (LAYOUT == CONTIG) {
+#pragma unroll 4
+      for (int idx = tid; idx < ROWS * E; idx += NT) {
+        const int r = idx / E, el = idx % E;
+        if (r < rows_valid && e0 + el < n)
+          lds[lds_at(r, el)] = ld_as_double(in + r * args.pstride + (e0 + el));
+      }
+    } else {
+#pragma unroll 4
+      for (int idx = tid; idx < ROWS * E; idx += NT) {
+        const int r = idx % ROWS, el = idx / ROWS;
+        if (r < rows_valid && e0 + el < n)
+          lds[lds_at(r, el)] = ld_as_double(in + r + (int64_t)(e0 + el) * args.estride);
+      }
+    }
+    __syncthreads();
+    if (my_blk == b && active) {
+      const int el0 = my_e0 - e0;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int e = my_e0 + k;
+        if (e < n) {
+          const double c = lds[lds_at(ql + HALO, el0 + k)];
+          if (PROD == P_ADI) {
+            const double up = lds[lds_at(ql, el0 + k)];
+            const double dn = lds[lds_at(ql + 2, el0 + k)];
+            f[k] = (e == 0 || e == n - 1) ? c : args.s1 * (up + dn) + args.s2 * c;
+          } else {
+            f[k] = c;
+          }
+        }
+      }
+    }
+    if (PROD == P_COMPACT && active) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const int e = (i < 3) ? i : n + 2 - i;
+        if (e >= e0 && e < e0 + E) bnd[i] = lds[lds_at(ql + HALO, e - e0)];
+      }
+    }
+  }
+
+  // ---------------- right-hand side -----------------------------------------
+  double d[R];
+  if (PROD == P_COMPACT) {
+    const double fl = shup<W>(f[R - 1], 1);      // row my_e0 - 1
+    const double fr = shdn<W>(f[0], 1);          // row my_e0 + R
+    const double b0 = args.s2 * (-5.0 * bnd[0] + 4.0 * bnd[1] + bnd[2]);
+    const double bn = -args.s2 * (-5.0 * bnd[3] + 4.0 * bnd[4] + bnd[5]);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int e = my_e0 + k;
+      const double left = (k > 0) ? f[k - 1] : fl;
+      const double right = (k + 1 < R) ? f[k + 1] : fr;
+      const double in_v = args.s1 * (right - left);
+      d[k] = (e == 0) ? b0 : ((e == n - 1) ? bn : ((e < n) ? in_v : 0.0));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) d[k] = f[k];
+  }
+
+  // ---------------- solve ----------------------------------------------------
+  double x[R];
+  if (PREC == PREC_F64) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[k] = d[k];
+    cr_solve_regs<double, R, W>(x, P.d, P.nlev, lane);
+  } else {
+    float s[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) s[k] = static_cast<float>(d[k]);
+    cr_solve_regs<float, R, W>(s, P.f, P.nlev, lane);
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[k] = static_cast<double>(s[k]);
+    for (int it = 0; it < args.sweeps; ++it) {
+      double r[R];
+      residual<R, W>(d, x, r, P, lane);
+#pragma unroll
+      for (int k = 0; k < R; ++k) s[k] = static_cast<float>(r[k]);
+      cr_solve_regs<float, R, W>(s, P.f, P.nlev, lane);
+#pragma unroll
+      for (int k = 0; k < R; ++k) x[k] += static_cast<double>(s[k]);
+    }
+  }
+
+  // ---------------- store: blocks back through LDS ---------------------------
+  for (int b = 0; b < nblk; ++b) {
+    const int e0 = b * E;
+    __syncthreads();
+    if (my_blk == b && active) {
+      const int el0 = my_e0 - e0;
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (my_e0 + k < n) lds[lds_at(ql + HALO, el0 + k)] = x[k];
+    }
+    __syncthreads();
+    if (LAYOUT == CONTIG) {
+#pragma unroll 4
+      for (int idx = tid; idx < TS * E; idx += NT) {
+        const int r = idx / E, el = idx % E;
+        if (p0 + r < nsys && e0 + el < n)
+          st_from_double(out + (r + HALO) * args.pstride + (e0 + el), lds[lds_at(r + HALO, el)]);
+      }
+    } else {
+#pragma unroll 4
+      for (int idx = tid; idx < TS * E; idx += NT) {
+        const int r = idx % TS, el = idx / TS;
+        if (p0 + r < nsys && e0 + el < n)
+          st_from_double(out + (r + HALO) + (int64_t)(e0 + el) * args.estride,
+                         lds[lds_at(r + HALO, el)]);
+      }
+    }
+  }
+}
+
+}  // namespace nt

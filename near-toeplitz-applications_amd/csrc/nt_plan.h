@@ -1,0 +1,72 @@
+// Near-Toeplitz cyclic-reduction plan: the level table computed once on the
+// host in FP64 and handed to every kernel by value (it lives in the kernel
+// argument segment, i.e. constant memory read through the scalar cache).
+//
+// A near-Toeplitz matrix is the 7-tuple (b1, c1, ai, bi, ci, an, bn): first row
+// [b1, c1], constant interior rows (ai, bi, ci), last row [an, bn]
+// (reference neato.py:136-138).  One CR reduction keeps the odd equations of a
+// level; the kept equations again form a near-Toeplitz system, so each level
+// is described by seven numbers and its equation count m.  The reference
+// precomputes these per level for power-of-2 n (neato.py:199-274); this plan
+// does the same for any n >= 2 (an odd m eliminates the level's last
+// equation into the last kept one, which then carries the factor c/bL).
+#pragma once
+#include <cstdint>
+
+#define NT_MAX_LEVELS 12   // levels 0..11: enough for any n <= 4096
+#define NT_MAX_N 2048      // the register-resident CR holds at most 64 lanes x 32 rows
+
+// Factors of one level, in the arithmetic type T of the solve.
+template <typename T>
+struct NtLevel {
+  // forward reduction that starts from this level (reduction s uses level s-1)
+  T k1;    // a/b      left factor of an interior kept equation
+  T k2;    // c/b      right factor of an interior kept equation
+  T k1f;   // a/bF     left factor of the first kept equation (its left neighbour is row 0)
+  T kl;    // m even: aL/b (left factor of the kept last row); m odd: c/bL (right factor of row m-2)
+  // back substitution of this level's eliminated (even) equations
+  T a, c, ib;     // interior row: x = (d - a*xl - c*xr) * ib
+  T cf, ibf;      // row 0:        x = (d - cf*xr) * ibf
+  T al, ibl;      // row m-1 (m odd only): x = (d - al*xl) * ibl
+  int32_t m;      // equations at this level
+  int32_t pad_;
+};
+
+// Direct solve of the top system (m <= 3 equations).
+template <typename T>
+struct NtTop {
+  int32_t m;
+  int32_t pad_;
+  // m == 1: x0 = d0*t[0]
+  // m == 2: x0 = d0*t[0] - d1*t[1];  x1 = d1*t[2] - d0*t[3]   (Cramer, neato.py:78-93)
+  // m == 3: x1 = (d1 - t[0]*d0 - t[1]*d2)*t[2]; x0 = (d0 - t[3]*x1)*t[4]; x2 = (d2 - t[5]*x1)*t[6]
+  T t[7];
+};
+
+template <typename T>
+struct NtPlanT {
+  NtLevel<T> lv[NT_MAX_LEVELS];
+  NtTop<T> top;
+};
+
+// Everything a kernel needs about one matrix.  Passed by value.
+struct NtPlanK {
+  int32_t n;       // system size
+  int32_t nlev;    // number of reductions (top system is level nlev)
+  double b1, c1, ai, bi, ci, an, bn;   // level-0 matrix (FP64 residuals)
+  NtPlanT<double> d;                   // FP64 factors
+  NtPlanT<float> f;                    // the same factors rounded to FP32
+};
+
+// Host-side plan object behind the opaque C handle.
+struct nt_plan {
+  int64_t n;
+  double coeffs[7];
+  int nlevels;                       // number of CR levels including level 0
+  double lev[NT_MAX_LEVELS][8];      // m, bF, cF, a, b, c, aL, bL per level
+  NtPlanK k;
+};
+
+// Fill `p` for an n x n near-Toeplitz matrix.  Returns 0, or a negative code
+// with a message in `err` (zero pivot, n out of range, non-finite input).
+int nt_plan_build(nt_plan* p, int64_t n, const double coeffs[7], char* err, int errlen);

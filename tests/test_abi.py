@@ -1,0 +1,124 @@
+"""The C ABI without a GPU: the library loads, exports every declared symbol,
+and its host-side plan logic (level table, reference arrays, validation)
+agrees with the oracle.  No kernel is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import adi, cr
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "ntoeplitz.h")
+
+
+def _declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_header_symbol():
+    from ntamd import _lib
+    lib = _lib.load()
+    syms = _declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), f"libntoeplitz.so does not export {s}"
+    # the ctypes signature table covers the header exactly
+    assert sorted(_lib.SIGNATURES) == syms
+
+
+def test_version_and_error_string():
+    from ntamd import _lib
+    lib = _lib.load()
+    assert lib.nt_version() >= 100
+    h = ctypes.c_void_p()
+    co = (ctypes.c_double * 7)(1, 2, .25, 1, .25, 2, 1)
+    rc = lib.nt_plan_create(ctypes.c_int64(1), co, ctypes.byref(h))
+    assert rc == _lib.NT_EINVAL
+    assert b"out of range" in lib.nt_last_error()
+    rc = lib.nt_plan_create(ctypes.c_int64(8), co, ctypes.byref(h))
+    assert rc == _lib.NT_OK and h.value
+    assert lib.nt_last_error() == b""
+    lib.nt_plan_destroy(h)
+
+
+COEFFS = {
+    "compact": (1.0, 2.0, 0.25, 1.0, 0.25, 2.0, 1.0),
+    "adi": adi.pr_coeffs(0.5),
+    "general": (2.0, 0.3, 0.5, 3.0, 0.7, 0.2, 1.7),
+}
+
+
+@pytest.mark.parametrize("name", sorted(COEFFS))
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 7, 8, 13, 64, 96, 100, 511, 512, 1000, 2048, 4096])
+def test_plan_levels_match_oracle(name, n):
+    from ntamd import Plan, NtSingular
+    if name == "compact" and n == 3:
+        with pytest.raises(NtSingular):           # det [[1,2,0],[.25,1,.25],[0,2,1]] = 0
+            Plan(n, COEFFS[name])
+        return
+    p = Plan(n, COEFFS[name])
+    lv = cr.level_systems(n, COEFFS[name])
+    got = p.levels()
+    assert got.shape == (len(lv), 8)
+    want = np.array([[L[k] for k in ("m", "bF", "cF", "a", "b", "c", "aL", "bL")] for L in lv])
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("name", sorted(COEFFS))
+@pytest.mark.parametrize("n", [4, 8, 64, 512, 2048])
+def test_plan_reference_arrays_match_neato(name, n):
+    from ntamd import Plan
+    got = Plan(n, COEFFS[name]).reference_arrays()
+    want = cr.reference_coefficients(n, COEFFS[name])
+    for g, w in zip(got, want):
+        np.testing.assert_allclose(g, w, rtol=1e-14, atol=1e-300)
+
+
+def test_plan_rejects_bad_input():
+    from ntamd import Plan, NtInvalidArgument, NtSingular
+    with pytest.raises(NtInvalidArgument):
+        Plan(1, COEFFS["compact"])
+    with pytest.raises(NtInvalidArgument):
+        Plan(5000, COEFFS["compact"])
+    with pytest.raises(NtInvalidArgument):
+        Plan(8, (1, 2, float("nan"), 1, 1, 1, 1))
+    with pytest.raises(NtSingular):
+        Plan(8, (0.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0))      # b1 = 0 pivot
+    with pytest.raises(NtSingular):
+        Plan(3, COEFFS["compact"])                        # the 3x3 compact matrix is singular
+    with pytest.raises(NtInvalidArgument):
+        Plan(8, (1, 2, 3))
+
+
+def test_reference_arrays_need_power_of_two():
+    from ntamd import Plan, NtInvalidArgument
+    with pytest.raises(NtInvalidArgument):
+        Plan(96, COEFFS["compact"]).reference_arrays()
+
+
+def test_tensor_layer_validates_before_any_launch():
+    import torch
+    from ntamd import Plan, core, NtInvalidArgument
+    p = Plan(8, COEFFS["compact"])
+    cpu = torch.zeros(4, 8, dtype=torch.float64)
+    with pytest.raises(NtInvalidArgument):
+        core.solve(p, cpu)                       # not a device tensor
+    with pytest.raises(TypeError):
+        core.solve(p, np.zeros(8))
+    with pytest.raises(NtInvalidArgument):
+        core.precision_code("fp16")
+
+
+def test_dropin_constructor_checks():
+    from neato import NearToeplitzSolver
+    with pytest.raises(ValueError):
+        NearToeplitzSolver(1, 4, COEFFS["compact"])
+    with pytest.raises(ValueError):
+        NearToeplitzSolver(4096, 4, COEFFS["compact"])
+    s = NearToeplitzSolver(96, 10, COEFFS["compact"])   # non power of two is accepted
+    assert s.plan.nlevels == len(cr.level_systems(96, COEFFS["compact"]))
