@@ -42,6 +42,15 @@ const char* nt_last_error(void);
 /* Library ABI version (major*100 + minor). */
 int nt_version(void);
 
+/* ---- launch tuning (no reference counterpart: the reference hard-codes one
+ * block per system, neato.py:181-183) --------------------------------------
+ * Launch-shape variant used for lines of length n (bucketed by ceil(log2 n))
+ * in layout 0 (contiguous lines) or 1 (strided lines).  Process-wide; set by
+ * the autotuner (ntamd/autotune.py) from its JSON cache.  Every variant gives
+ * the same results; only the launch shape changes. */
+int nt_tune_set(int layout, int64_t n, int variant);
+int nt_tune_get(int layout, int64_t n, int* variant, int* nvariants);
+
 /* ---- plans ---------------------------------------------------------------
  * Replaces NearToeplitzSolver.__init__(n, nrhs, coeffs) (neato.py:131-166):
  * coeffs = (b1, c1, ai, bi, ci, an, bn) as in neato.py:136-138.  The level
@@ -110,6 +119,31 @@ int nt_transpose(const void* a, void* b, int64_t N, void* stream);
 int nt_adi_prepare(const void* u, void* work, int64_t N, void* stream);
 int nt_adi_steps(const nt_plan* plan, void* u, void* work, int64_t N, double r, int nsteps,
                  int precision, int sweeps, void* stream);
+
+/* ---- one-GPU partitioned (superposition) solve ------------------------------
+ * The reference's distributed solver cuts each line into P subdomains over P
+ * MPI ranks: local solves, secondary solutions (parallelcompact/misc.py:21-46),
+ * a 2P-unknown reduced system gathered to one rank and solved there
+ * (parallelcompact/reduced.py:11-196), a correction sum (parallelcompact/sum.py).
+ * Here the same method runs on one GPU as one launch per batch of lines.
+ * A plan fixes n = P*m, P <= 16 and the 7-tuple; it precomputes the subdomain
+ * level tables, xU/xL and the reduced system's factors on the host in FP64
+ * and keeps them in one device allocation (made at creation, on the current
+ * device; none if no device is present). */
+typedef struct nt_part_plan nt_part_plan;
+int nt_part_plan_create(int64_t n, int P, const double coeffs[7], nt_part_plan** out);
+int nt_part_plan_destroy(nt_part_plan* plan);
+/* Host copies of xU, xL of subdomain p (misc.py:21-46 solve_secondary_systems). */
+int nt_part_plan_secondary(const nt_part_plan* plan, int p, double* xU, double* xL, int cap);
+/* producer 0: solve `lines` contiguous FP64 lines of length n in place (in == out allowed);
+ * producer 1: compact derivative along contiguous lines, p1 = dx
+ *             (parallelcompact rhs.py + solve + get_params + sum_solutions of
+ *             run-parallel-compact.py:79-85, one GPU);
+ * producer 2: ADI x-implicit half step of an n x n field in -> out (lines == n),
+ *             p1 = r; rows 0 and n-1 of out are not written.
+ * p2 is reserved (pass 0). */
+int nt_part_solve_lines(const nt_part_plan* plan, int producer, const void* in, void* out,
+                        int64_t lines, double p1, double p2, void* stream);
 
 #ifdef __cplusplus
 }

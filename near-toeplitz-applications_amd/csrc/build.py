@@ -21,10 +21,15 @@ SOURCES = [
     "nt_plan.cpp",
     "nt_api.hip",
     "nt_aux_kernels.hip",
-    "nt_k_ident.hip",
-    "nt_k_compact64.hip",
-    "nt_k_compact32.hip",
-    "nt_k_adi.hip",
+    "nt_k_ident_contig.hip",
+    "nt_k_ident_strided.hip",
+    "nt_k_compact64_contig.hip",
+    "nt_k_compact64_strided.hip",
+    "nt_k_compact32_contig.hip",
+    "nt_k_compact32_strided.hip",
+    "nt_k_adi_contig.hip",
+    "nt_k_adi_strided.hip",
+    "nt_part.hip",
 ]
 
 ARCH = os.environ.get("NT_OFFLOAD_ARCH", "gfx950")

@@ -13,14 +13,19 @@
 
 #include "../../include/ntoeplitz.h"
 #include "nt_aux.h"
+#include "nt_error.h"
 #include "nt_launch.h"
 #include "nt_plan.h"
 
 namespace {
-
 thread_local char g_err[512] = "";
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+int nt::set_error(int code, const char* fmt, ...) {
+  if (code == NT_OK) {
+    g_err[0] = '\0';
+    return NT_OK;
+  }
   va_list ap;
   va_start(ap, fmt);
   std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -28,10 +33,25 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-int ok() {
-  g_err[0] = '\0';
-  return NT_OK;
+namespace {
+// launch-variant table: [layout][ceil(log2 n)]
+int g_tune[2][13] = {};
+
+int log2_bucket(int n) {
+  int b = 0;
+  while ((1 << b) < n && b < 12) ++b;
+  return b;
 }
+}  // namespace
+
+int nt::tune_get(int layout, int n) { return g_tune[layout & 1][log2_bucket(n)]; }
+void nt::tune_set(int layout, int n, int variant) { g_tune[layout & 1][log2_bucket(n)] = variant; }
+
+namespace {
+
+#define fail nt::set_error
+
+int ok() { return nt::set_error(NT_OK, "%s", ""); }
 
 int hip_status(hipError_t e, const char* what) {
   if (e == hipSuccess) return ok();
@@ -99,6 +119,24 @@ extern "C" {
 const char* nt_last_error(void) { return g_err; }
 
 int nt_version(void) { return 100; }
+
+int nt_tune_set(int layout, int64_t n, int variant) {
+  if (layout != 0 && layout != 1) return fail(NT_EINVAL, "nt_tune_set: layout must be 0 or 1");
+  if (n < 2 || n > NT_MAX_N) return fail(NT_EINVAL, "nt_tune_set: n out of range");
+  if (variant < 0 || variant >= nt::kNumVariants[layout])
+    return fail(NT_EINVAL, "nt_tune_set: variant %d out of range [0, %d)", variant,
+                nt::kNumVariants[layout]);
+  nt::tune_set(layout, static_cast<int>(n), variant);
+  return ok();
+}
+
+int nt_tune_get(int layout, int64_t n, int* variant, int* nvariants) {
+  if (layout != 0 && layout != 1) return fail(NT_EINVAL, "nt_tune_get: layout must be 0 or 1");
+  if (n < 2 || n > NT_MAX_N) return fail(NT_EINVAL, "nt_tune_get: n out of range");
+  if (variant) *variant = nt::tune_get(layout, static_cast<int>(n));
+  if (nvariants) *nvariants = nt::kNumVariants[layout];
+  return ok();
+}
 
 int nt_plan_create(int64_t n, const double coeffs[7], nt_plan** out) {
   if (!out || !coeffs) return fail(NT_EINVAL, "nt_plan_create: null argument");

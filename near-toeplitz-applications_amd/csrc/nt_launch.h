@@ -14,6 +14,14 @@ namespace nt {
 template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
 hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 
+// Launch-shape variants, chosen per (layout, ceil(log2 n)) by the tuning table
+// (nt_tune_set; defaults 0).  CONTIG: 0 LDS-staged tile of 4 lines; 1 direct
+// chunk loads, 4 waves per group; 2 direct, 1 wave per group.  STRIDED: 0 tile
+// of 16 lines (16 lanes per line); 1 tile of 32 lines.
+constexpr int kNumVariants[2] = {3, 2};
+int tune_get(int layout, int n);
+void tune_set(int layout, int n, int variant);
+
 #define NT_GO(R_, W_, WAVES_, E_)                                                          \
   do {                                                                                     \
     constexpr int TS_ = (WAVES_) * (64 / (W_));                                            \
@@ -28,10 +36,61 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
     return hipGetLastError();                                                              \
   } while (0)
 
+// direct (LDS-free) contiguous-line kernel; one segment of W lanes per line
+#define NT_GOD(R_, W_, WAVES_)                                                             \
+  do {                                                                                     \
+    constexpr int TS_ = (WAVES_) * (64 / (W_));                                            \
+    const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
+    a.tiles = (nsys_ + TS_ - 1) / TS_;                                                     \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    const int64_t grid_ = outer * a.tiles;                                                 \
+    if (grid_ <= 0) return hipSuccess;                                                     \
+    if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
+    hipLaunchKernelGGL((line_kernel_direct<TIn, TOut, R_, W_, WAVES_, PROD, PREC>),        \
+                       dim3(static_cast<unsigned>(grid_)), dim3((WAVES_) * 64), 0, st, a); \
+    return hipGetLastError();                                                              \
+  } while (0)
+
+// 16-byte chunk accesses need 16-byte aligned line starts
+inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
+  const uintptr_t i = reinterpret_cast<uintptr_t>(a.in), o = reinterpret_cast<uintptr_t>(a.out);
+  return (i % 16 == 0) && (o % 16 == 0) && ((a.pstride * in_size) % 16 == 0) &&
+         ((a.pstride * out_size) % 16 == 0) && a.estride == 1;
+}
+
 template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
 hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
   const int n = a.plan.n;
-  if (LAYOUT == CONTIG) {
+  const int variant = tune_get(LAYOUT, n);
+  if constexpr (LAYOUT == CONTIG) {
+   if (variant > 0 && n > 64 && direct_ok(a, sizeof(TIn), sizeof(TOut))) {
+    if (variant == 1) {
+      if (n <= 128) NT_GOD(2, 64, 4);
+      if (n <= 256) NT_GOD(4, 64, 4);
+      if (n <= 512) NT_GOD(8, 64, 4);
+      if (n <= 1024) NT_GOD(16, 64, 4);
+      if (n <= 2048) NT_GOD(32, 64, 4);
+    } else {
+      if (n <= 128) NT_GOD(2, 64, 1);
+      if (n <= 256) NT_GOD(4, 64, 1);
+      if (n <= 512) NT_GOD(8, 64, 1);
+      if (n <= 1024) NT_GOD(16, 64, 1);
+      if (n <= 2048) NT_GOD(32, 64, 1);
+    }
+   }
+  }
+  if constexpr (LAYOUT == STRIDED) {
+   if (variant == 1) {
+    // 32 adjacent lines per tile (256-byte FP64 rows)
+    if (n <= 16) NT_GO(1, 16, 8, 16);
+    if (n <= 32) NT_GO(2, 16, 8, 32);
+    if (n <= 64) NT_GO(4, 16, 8, 64);
+    if (n <= 128) NT_GO(8, 16, 8, 64);
+    if (n <= 256) NT_GO(16, 16, 8, 64);
+    if (n <= 512) NT_GO(32, 16, 8, 64);
+   }
+  }
+  if constexpr (LAYOUT == CONTIG) {
     // one system per 64-lane wavefront (16 lanes for tiny systems), 4 waves
     if (n <= 16) NT_GO(1, 16, 4, 16);
     if (n <= 64) NT_GO(1, 64, 4, 64);

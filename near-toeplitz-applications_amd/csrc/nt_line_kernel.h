@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "nt_cr.h"
 #include "nt_plan.h"
@@ -73,6 +74,54 @@ __device__ __forceinline__ void residual(const double (&d)[R], const double (&x)
       ax = P.ai * left + P.bi * x[k] + P.ci * right;
     }
     r[k] = (e < n) ? d[k] - ax : 0.0;
+  }
+}
+
+
+// Compact RHS (compact/rhs.py:21-36) of my chunk from the chunk values f, the
+// neighbour rows fl (row my_e0-1) and fr (row my_e0+R) and the six boundary
+// values bnd = f[0], f[1], f[2], f[n-1], f[n-2], f[n-3].
+template <int R>
+__device__ __forceinline__ void compact_rhs_chunk(const double (&f)[R], double fl, double fr,
+                                                  const double (&bnd)[6], int my_e0, int n,
+                                                  double s1, double s2, double (&d)[R]) {
+  const double b0 = s2 * (-5.0 * bnd[0] + 4.0 * bnd[1] + bnd[2]);
+  const double bn = -s2 * (-5.0 * bnd[3] + 4.0 * bnd[4] + bnd[5]);
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int e = my_e0 + k;
+    const double left = (k > 0) ? f[k - 1] : fl;
+    const double right = (k + 1 < R) ? f[k + 1] : fr;
+    const double in_v = s1 * (right - left);
+    d[k] = (e == 0) ? b0 : ((e == n - 1) ? bn : ((e < n) ? in_v : 0.0));
+  }
+}
+
+// Solve A x = d for my chunk: FP64 CR, or FP32 CR + `sweeps` FP64-residual
+// refinement sweeps (x += A32^{-1} (d - A x), residual in FP64).
+template <int R, int W, int PREC>
+__device__ __forceinline__ void solve_chunk(const double (&d)[R], double (&x)[R], const NtPlanK& P,
+                                            int lane, int sweeps) {
+  if (PREC == PREC_F64) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[k] = d[k];
+    cr_solve_regs<double, R, W>(x, P.d, P.nlev, lane);
+  } else {
+    float s[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) s[k] = static_cast<float>(d[k]);
+    cr_solve_regs<float, R, W>(s, P.f, P.nlev, lane);
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[k] = static_cast<double>(s[k]);
+    for (int it = 0; it < sweeps; ++it) {
+      double r[R];
+      residual<R, W>(d, x, r, P, lane);
+#pragma unroll
+      for (int k = 0; k < R; ++k) s[k] = static_cast<float>(r[k]);
+      cr_solve_regs<float, R, W>(s, P.f, P.nlev, lane);
+#pragma unroll
+      for (int k = 0; k < R; ++k) x[k] += static_cast<double>(s[k]);
+    }
   }
 }
 
@@ -166,49 +215,18 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args) {
     }
   }
 
-  // ---------------- right-hand side -----------------------------------------
+  // ---------------- right-hand side and solve ---------------------------------
   double d[R];
   if (PROD == P_COMPACT) {
     const double fl = shup<W>(f[R - 1], 1);      // row my_e0 - 1
     const double fr = shdn<W>(f[0], 1);          // row my_e0 + R
-    const double b0 = args.s2 * (-5.0 * bnd[0] + 4.0 * bnd[1] + bnd[2]);
-    const double bn = -args.s2 * (-5.0 * bnd[3] + 4.0 * bnd[4] + bnd[5]);
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int e = my_e0 + k;
-      const double left = (k > 0) ? f[k - 1] : fl;
-      const double right = (k + 1 < R) ? f[k + 1] : fr;
-      const double in_v = args.s1 * (right - left);
-      d[k] = (e == 0) ? b0 : ((e == n - 1) ? bn : ((e < n) ? in_v : 0.0));
-    }
+    compact_rhs_chunk<R>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
   } else {
 #pragma unroll
     for (int k = 0; k < R; ++k) d[k] = f[k];
   }
-
-  // ---------------- solve ----------------------------------------------------
   double x[R];
-  if (PREC == PREC_F64) {
-#pragma unroll
-    for (int k = 0; k < R; ++k) x[k] = d[k];
-    cr_solve_regs<double, R, W>(x, P.d, P.nlev, lane);
-  } else {
-    float s[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) s[k] = static_cast<float>(d[k]);
-    cr_solve_regs<float, R, W>(s, P.f, P.nlev, lane);
-#pragma unroll
-    for (int k = 0; k < R; ++k) x[k] = static_cast<double>(s[k]);
-    for (int it = 0; it < args.sweeps; ++it) {
-      double r[R];
-      residual<R, W>(d, x, r, P, lane);
-#pragma unroll
-      for (int k = 0; k < R; ++k) s[k] = static_cast<float>(r[k]);
-      cr_solve_regs<float, R, W>(s, P.f, P.nlev, lane);
-#pragma unroll
-      for (int k = 0; k < R; ++k) x[k] += static_cast<double>(s[k]);
-    }
-  }
+  solve_chunk<R, W, PREC>(d, x, P, lane, args.sweeps);
 
   // ---------------- store: blocks back through LDS ---------------------------
   for (int b = 0; b < nblk; ++b) {
@@ -238,6 +256,114 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args) {
       }
     }
   }
+}
+
+
+// ---- contiguous lines without LDS staging ---------------------------------------
+// Chunk loads/stores: 16-byte accesses for whole, aligned chunks (the host only
+// picks this kernel when lines start 16-byte aligned and n is a multiple of the
+// vector width), element accesses for the partial chunk at the end of a line.
+template <typename T, int R>
+__device__ __forceinline__ void load_chunk(const T* __restrict__ line, int e0, int n, double (&f)[R]) {
+  constexpr int V = 16 / sizeof(T);
+  if (R % V == 0 && e0 + R <= n) {
+    using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    const VT* p = reinterpret_cast<const VT*>(line + e0);
+#pragma unroll
+    for (int v = 0; v < R / V; ++v) {
+      const VT t = p[v];
+      if constexpr (sizeof(T) == 8) {
+        f[v * 2] = t.x;
+        f[v * 2 + 1] = t.y;
+      } else {
+        f[v * 4] = t.x;
+        f[v * 4 + 1] = t.y;
+        f[v * 4 + 2] = t.z;
+        f[v * 4 + 3] = t.w;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) f[k] = (e0 + k < n) ? static_cast<double>(line[e0 + k]) : 0.0;
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void store_chunk(T* __restrict__ line, int e0, int n, const double (&x)[R]) {
+  constexpr int V = 16 / sizeof(T);
+  if (R % V == 0 && e0 + R <= n) {
+    using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    VT* p = reinterpret_cast<VT*>(line + e0);
+#pragma unroll
+    for (int v = 0; v < R / V; ++v) {
+      VT t;
+      if constexpr (sizeof(T) == 8) {
+        t.x = x[v * 2];
+        t.y = x[v * 2 + 1];
+      } else {
+        t.x = static_cast<float>(x[v * 4]);
+        t.y = static_cast<float>(x[v * 4 + 1]);
+        t.z = static_cast<float>(x[v * 4 + 2]);
+        t.w = static_cast<float>(x[v * 4 + 3]);
+      }
+      p[v] = t;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      if (e0 + k < n) line[e0 + k] = static_cast<T>(x[k]);
+  }
+}
+
+// Every lane reads its R consecutive rows directly; the lanes of a segment
+// together read the line's contiguous bytes, neighbour rows cross lanes by
+// shuffles, and there is no LDS and no barrier.  Each segment (W lanes) owns
+// one line, a workgroup TS = WAVES*64/W lines.
+template <typename TIn, typename TOut, int R, int W, int WAVES, int PROD, int PREC>
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_direct(const LineArgs args) {
+  constexpr int SPW = 64 / W;
+  constexpr int TS = WAVES * SPW;
+  constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
+  const NtPlanK& P = args.plan;
+  const int n = P.n;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int seg = (tid & 63) / W;
+  const int lane = tid & (W - 1);
+  const int64_t nsys = args.phys - 2 * HALO;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * TS + wave * SPW + seg;
+  if (q >= nsys) return;   // the whole segment leaves together; no barrier follows
+  const int64_t row = q + HALO;
+  const TIn* __restrict__ line = static_cast<const TIn*>(args.in) + row * args.pstride;
+  TOut* __restrict__ oline = static_cast<TOut*>(args.out) + row * args.pstride;
+  const int my_e0 = lane * R;
+
+  double f[R];
+  load_chunk<TIn, R>(line, my_e0, n, f);
+  double d[R];
+  if (PROD == P_COMPACT) {
+    double bnd[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) bnd[i] = static_cast<double>(line[(i < 3) ? i : n + 2 - i]);
+    const double fl = shup<W>(f[R - 1], 1);
+    const double fr = shdn<W>(f[0], 1);
+    compact_rhs_chunk<R>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+  } else if (PROD == P_ADI) {
+    double up[R], dn[R];
+    load_chunk<TIn, R>(line - args.pstride, my_e0, n, up);
+    load_chunk<TIn, R>(line + args.pstride, my_e0, n, dn);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int e = my_e0 + k;
+      d[k] = (e == 0 || e >= n - 1) ? f[k] : args.s1 * (up[k] + dn[k]) + args.s2 * f[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) d[k] = f[k];
+  }
+  double x[R];
+  solve_chunk<R, W, PREC>(d, x, P, lane, args.sweeps);
+  store_chunk<TOut, R>(oline, my_e0, n, x);
 }
 
 }  // namespace nt

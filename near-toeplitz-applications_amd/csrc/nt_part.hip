@@ -1,0 +1,276 @@
+// Partitioned-solve plans and launches (C ABI part; see nt_part_kernel.h).
+//
+// Plan time (host, FP64, once): the three subdomain level tables, the
+// secondary solutions xU/xL of parallelcompact/misc.py:21-46 (here by a Thomas
+// sweep on the host instead of SciPy), the reduced system of
+// parallelcompact/reduced.py:163-178 and its Thomas factors.  Everything is
+// copied to one device allocation owned by the plan; the reference instead
+// recomputes the secondary solutions and reallocates buffers on every call
+// (run-parallel-compact.py:82, reduced.py:136-145).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/ntoeplitz.h"
+#include "nt_error.h"
+#include "nt_part_kernel.h"
+#include "nt_plan.h"
+
+struct nt_part_plan {
+  int64_t n;
+  int P, m;
+  double coeffs[7];
+  std::vector<double> xu, xl;    // host copies [3][m]
+  nt::NtPartDev host;            // host image of the device header
+  void* dev;                     // device allocation: header + xU + xL
+  int device;
+  size_t bytes;
+};
+
+namespace {
+
+// Thomas solve of a tridiagonal system (a sub, b diag, c super), no pivoting.
+bool thomas(int n, const double* a, const double* b, const double* c, const double* d, double* x) {
+  std::vector<double> cp(n), dp(n);
+  if (b[0] == 0.0) return false;
+  cp[0] = c[0] / b[0];
+  dp[0] = d[0] / b[0];
+  for (int i = 1; i < n; ++i) {
+    const double den = b[i] - a[i] * cp[i - 1];
+    if (den == 0.0 || !std::isfinite(den)) return false;
+    cp[i] = c[i] / den;
+    dp[i] = (d[i] - a[i] * dp[i - 1]) / den;
+  }
+  x[n - 1] = dp[n - 1];
+  for (int i = n - 2; i >= 0; --i) x[i] = dp[i] - cp[i] * x[i + 1];
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nt_part_plan_create(int64_t n, int P, const double coeffs[7], nt_part_plan** out) {
+  if (!out || !coeffs) return nt::set_error(NT_EINVAL, "%s", "nt_part_plan_create: null argument");
+  *out = nullptr;
+  if (P < 1 || P > NT_PART_MAX_P) return nt::set_error(NT_EINVAL, "%s", "nt_part_plan_create: P must be in [1, 16]");
+  if (n < 2 || n > NT_PART_MAX_N || n % P != 0)
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_plan_create: need 2 <= n <= 2048 and n divisible by P");
+  const int m = static_cast<int>(n / P);
+  if (m < 4 || m > 64 * 32)
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_plan_create: subdomains need 4 <= m <= 2048 points");
+  nt_part_plan* pp = new (std::nothrow) nt_part_plan;
+  if (!pp) return nt::set_error(NT_EINVAL, "%s", "nt_part_plan_create: out of host memory");
+  pp->n = n;
+  pp->P = P;
+  pp->m = m;
+  pp->dev = nullptr;
+  std::memcpy(pp->coeffs, coeffs, sizeof(pp->coeffs));
+  std::memset(&pp->host, 0, sizeof(pp->host));
+  const double b1 = coeffs[0], c1 = coeffs[1], ai = coeffs[2], bi = coeffs[3], ci = coeffs[4],
+               an = coeffs[5], bn = coeffs[6];
+  // subdomain kinds: 0 first (keeps b1, c1), 1 interior (principal block), 2 last (keeps an, bn)
+  double kinds[3][7] = {{b1, c1, ai, bi, ci, ai, bi},
+                        {bi, ci, ai, bi, ci, ai, bi},
+                        {bi, ci, ai, bi, ci, an, bn}};
+  if (P == 1) std::memcpy(kinds[0], coeffs, sizeof(kinds[0]));
+  pp->xu.assign(3 * m, 0.0);
+  pp->xl.assign(3 * m, 0.0);
+  char msg[256];
+  for (int k = 0; k < 3; ++k) {
+    nt_plan tmp;
+    const int rc = nt_plan_build(&tmp, m, kinds[k], msg, sizeof(msg));
+    if (rc != 0) {
+      delete pp;
+      char buf[400];
+      std::snprintf(buf, sizeof(buf), "nt_part_plan_create: subdomain kind %d: %s", k, msg);
+      return nt::set_error(rc == -2 ? NT_ESINGULAR : NT_EINVAL, "%s", buf);
+    }
+    pp->host.plan[k] = tmp.k.d;
+    pp->host.nlev = tmp.k.nlev;
+    // secondary solutions (misc.py:21-46): rU = -a[0] e_0, rL = -c[-1] e_{m-1}
+    std::vector<double> a(m, ai), b(m, bi), c(m, ci), rU(m, 0.0), rL(m, 0.0);
+    a[0] = (k == 0) ? 0.0 : ai;
+    b[0] = kinds[k][0];
+    c[0] = kinds[k][1];
+    a[m - 1] = kinds[k][5];
+    b[m - 1] = kinds[k][6];
+    c[m - 1] = (k == 2 || P == 1) ? 0.0 : ci;
+    if (P == 1) a[0] = 0.0;
+    rU[0] = -a[0];
+    rL[m - 1] = -c[m - 1];
+    if (!thomas(m, a.data(), b.data(), c.data(), rU.data(), &pp->xu[k * m]) ||
+        !thomas(m, a.data(), b.data(), c.data(), rL.data(), &pp->xl[k * m])) {
+      delete pp;
+      return nt::set_error(NT_ESINGULAR, "%s", "nt_part_plan_create: zero pivot in a secondary solve");
+    }
+  }
+  // reduced system (reduced.py:163-178) from [x[0], x[-1]] of every subdomain
+  const int N2 = 2 * P;
+  std::vector<double> ra(N2, 0.0), rb(N2, 0.0), rc(N2, 0.0);
+  for (int p = 0; p < P; ++p) {
+    const int k = (P == 1) ? 0 : (p == 0 ? 0 : (p == P - 1 ? 2 : 1));
+    const double xu0 = pp->xu[k * m], xum = pp->xu[k * m + m - 1];
+    const double xl0 = pp->xl[k * m], xlm = pp->xl[k * m + m - 1];
+    ra[2 * p] = -1.0;
+    rb[2 * p] = xu0;
+    rc[2 * p] = xl0;
+    ra[2 * p + 1] = xum;
+    rb[2 * p + 1] = xlm;
+    rc[2 * p + 1] = -1.0;
+  }
+  ra[0] = rc[0] = 0.0;
+  rb[0] = 1.0;
+  ra[N2 - 1] = rc[N2 - 1] = 0.0;
+  rb[N2 - 1] = 1.0;
+  ra[1] = 0.0;
+  rc[N2 - 2] = 0.0;
+  double c2 = 0.0;
+  for (int i = 0; i < N2; ++i) {
+    const double den = rb[i] - (i > 0 ? ra[i] * c2 : 0.0);
+    if (den == 0.0 || !std::isfinite(den)) {
+      delete pp;
+      return nt::set_error(NT_ESINGULAR, "%s", "nt_part_plan_create: zero pivot in the reduced system");
+    }
+    pp->host.red_a[i] = ra[i];
+    pp->host.red_ib[i] = 1.0 / den;
+    c2 = rc[i] / den;
+    pp->host.red_c2[i] = c2;
+  }
+  pp->host.m = m;
+  pp->host.P = P;
+  pp->host.n = static_cast<int32_t>(n);
+  pp->host.b1 = b1; pp->host.c1 = c1; pp->host.ai = ai; pp->host.bi = bi;
+  pp->host.ci = ci; pp->host.an = an; pp->host.bn = bn;
+  // device copy (skipped without a HIP device: the host data stays queryable,
+  // solves then fail with a clear error)
+  pp->bytes = sizeof(nt::NtPartDev) + 6 * static_cast<size_t>(m) * sizeof(double);
+  pp->device = -1;
+  int devcount = 0;
+  if (hipGetDeviceCount(&devcount) == hipSuccess && devcount > 0) {
+    hipError_t e = hipGetDevice(&pp->device);
+    if (e == hipSuccess) e = hipMalloc(&pp->dev, pp->bytes);
+    if (e == hipSuccess)
+      e = hipMemcpy(pp->dev, &pp->host, sizeof(nt::NtPartDev), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      double* tail = reinterpret_cast<double*>(static_cast<char*>(pp->dev) + sizeof(nt::NtPartDev));
+      e = hipMemcpy(tail, pp->xu.data(), 3 * m * sizeof(double), hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(tail + 3 * m, pp->xl.data(), 3 * m * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+      if (pp->dev) (void)hipFree(pp->dev);
+      delete pp;
+      char buf[300];
+      std::snprintf(buf, sizeof(buf), "nt_part_plan_create: %s", hipGetErrorString(e));
+      return nt::set_error(NT_EHIP, "%s", buf);
+    }
+  } else {
+    (void)hipGetLastError();
+  }
+  *out = pp;
+  return nt::set_error(NT_OK, "%s", "");
+}
+
+int nt_part_plan_destroy(nt_part_plan* pp) {
+  if (pp) {
+    if (pp->dev) (void)hipFree(pp->dev);
+    delete pp;
+  }
+  return nt::set_error(NT_OK, "%s", "");
+}
+
+int nt_part_plan_secondary(const nt_part_plan* pp, int p, double* xU, double* xL, int cap) {
+  if (!pp || !xU || !xL) return nt::set_error(NT_EINVAL, "%s", "nt_part_plan_secondary: null argument");
+  if (p < 0 || p >= pp->P) return nt::set_error(NT_EINVAL, "%s", "nt_part_plan_secondary: bad subdomain");
+  if (cap < pp->m) return nt::set_error(NT_EINVAL, "%s", "nt_part_plan_secondary: buffer too small");
+  const int k = (pp->P == 1) ? 0 : (p == 0 ? 0 : (p == pp->P - 1 ? 2 : 1));
+  std::memcpy(xU, &pp->xu[k * pp->m], pp->m * sizeof(double));
+  std::memcpy(xL, &pp->xl[k * pp->m], pp->m * sizeof(double));
+  return nt::set_error(NT_OK, "%s", "");
+}
+
+int nt_part_solve_lines(const nt_part_plan* pp, int producer, const void* in, void* out,
+                        int64_t lines, double p1, double p2, void* stream) {
+  (void)p2;
+  if (!pp || !in || !out) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: null argument");
+  if (lines < 0) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: lines < 0");
+  int cur = -1;
+  if (!pp->dev) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: plan has no device copy (no HIP device at creation)");
+  if (hipGetDevice(&cur) != hipSuccess || cur != pp->device)
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: plan was created on another device");
+  nt::PartArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.dev = static_cast<const nt::NtPartDev*>(pp->dev);
+  const double* tail =
+      reinterpret_cast<const double*>(static_cast<const char*>(pp->dev) + sizeof(nt::NtPartDev));
+  a.xu = tail;
+  a.xl = tail + 3 * pp->m;
+  a.in = in;
+  a.out = out;
+  a.lstride = pp->n;
+  int64_t grid = lines;
+  if (producer == 0) {
+    a.lines = lines;
+  } else if (producer == 1) {
+    if (!(std::isfinite(p1) && p1 != 0.0)) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: dx");
+    if (in == out) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: f and out must not alias");
+    a.lines = lines;
+    a.s1 = 3.0 / (4.0 * p1);
+    a.s2 = 1.0 / (2.0 * p1);
+  } else if (producer == 2) {
+    // ADI x-implicit half step: `lines` = N rows of an N x N field, rows 1..N-2 solved
+    if (lines != pp->n) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: ADI needs lines == n");
+    if (in == out) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: u and work must not alias");
+    const double* c = pp->coeffs;
+    if (c[0] != 1.0 || c[1] != 0.0 || c[5] != 0.0 || c[6] != 1.0)
+      return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: ADI plan needs identity end rows");
+    a.lines = lines;
+    a.s1 = 0.5 * p1;
+    a.s2 = 1.0 - p1;
+    grid = lines - 2;
+  } else {
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: producer must be 0, 1 or 2");
+  }
+  if (grid <= 0) return nt::set_error(NT_OK, "%s", "");
+  if (grid > 0x7fffffffLL) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: too many lines");
+  const int m = pp->m;
+  int R = 1;
+  while (64 * R < m) R *= 2;
+  const dim3 g(static_cast<unsigned>(grid)), b(64 * pp->P);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define NT_PGO(RR)                                                                       \
+  case RR:                                                                               \
+    if (producer == 0)                                                                   \
+      hipLaunchKernelGGL((nt::part_kernel<RR, nt::P_IDENT>), g, b, 0, st, a);            \
+    else if (producer == 1)                                                              \
+      hipLaunchKernelGGL((nt::part_kernel<RR, nt::P_COMPACT>), g, b, 0, st, a);          \
+    else                                                                                 \
+      hipLaunchKernelGGL((nt::part_kernel<RR, nt::P_ADI>), g, b, 0, st, a);              \
+    break;
+  switch (R) {
+    NT_PGO(1)
+    NT_PGO(2)
+    NT_PGO(4)
+    NT_PGO(8)
+    NT_PGO(16)
+    NT_PGO(32)
+    default:
+      return nt::set_error(NT_EUNSUPPORTED, "%s", "nt_part_solve_lines: subdomain too large");
+  }
+#undef NT_PGO
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char buf[300];
+    std::snprintf(buf, sizeof(buf), "nt_part_solve_lines: %s", hipGetErrorString(e));
+    return nt::set_error(NT_EHIP, "%s", buf);
+  }
+  return nt::set_error(NT_OK, "%s", "");
+}
+
+}  // extern "C"

@@ -34,6 +34,8 @@ _intp = ctypes.POINTER(ctypes.c_int)
 SIGNATURES = {
     "nt_last_error": (ctypes.c_char_p, []),
     "nt_version": (_int, []),
+    "nt_tune_set": (_int, [_int, _i64, _int]),
+    "nt_tune_get": (_int, [_int, _i64, _intp, _intp]),
     "nt_plan_create": (_int, [_i64, _dp, _pp]),
     "nt_plan_destroy": (_int, [_vp]),
     "nt_plan_info": (_int, [_vp, _i64p, _intp]),
@@ -49,6 +51,10 @@ SIGNATURES = {
     "nt_transpose": (_int, [_vp, _vp, _i64, _vp]),
     "nt_adi_prepare": (_int, [_vp, _vp, _i64, _vp]),
     "nt_adi_steps": (_int, [_vp, _vp, _vp, _i64, _dbl, _int, _int, _int, _vp]),
+    "nt_part_plan_create": (_int, [_i64, _int, _dp, _pp]),
+    "nt_part_plan_destroy": (_int, [_vp]),
+    "nt_part_plan_secondary": (_int, [_vp, _int, _dp, _dp, _int]),
+    "nt_part_solve_lines": (_int, [_vp, _int, _vp, _vp, _i64, _dbl, _dbl, _vp]),
 }
 
 
