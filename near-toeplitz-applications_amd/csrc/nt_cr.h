@@ -14,8 +14,17 @@
 // rows 1, 3, 5, ..., every later one keeps rows 2^s-1 mod 2^s, the top system is
 // solved directly and back substitution walks the levels down to the even rows
 // of level 0 (the reference's "last substitution" at store, neato.py:112-123).
+//
+// The code is branch-free: every per-row decision is a select, so a wavefront
+// never splits.  With FULL (n == W*R, n a power of two, every level even) the
+// level structure is known at compile time and only the first and last lanes
+// select special factors.  Distance-1 shuffles use DPP lane shifts (no LDS
+// traffic); longer distances use ds_bpermute; broadcasts from a known lane use
+// v_readlane when a segment is the whole wavefront.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 #include "nt_plan.h"
 
@@ -30,85 +39,190 @@ struct Log2<1> {
   static constexpr int value = 0;
 };
 
-// Segment-local shuffles: lanes whose source falls outside the segment read
-// their own value (HIP semantics of __shfl_up/__shfl_down with a width).
+// ---- cross-lane moves --------------------------------------------------------------
+// 32-bit halves of a double (DPP and readlane move 32-bit registers)
+__device__ __forceinline__ int lo32(double v) { return __double2loint(v); }
+__device__ __forceinline__ int hi32(double v) { return __double2hiint(v); }
+__device__ __forceinline__ double mkdbl(int hi, int lo) { return __hiloint2double(hi, lo); }
+
+// DPP control words (GFX9 encoding)
+constexpr int DPP_ROW_SHL1 = 0x101;
+constexpr int DPP_ROW_SHR1 = 0x111;
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp32(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(dpp32<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  return mkdbl(dpp32<CTRL>(hi32(v)), dpp32<CTRL>(lo32(v)));
+}
+
+// value of lane+1 / lane-1.  Lanes whose neighbour is outside the segment get
+// an unspecified value (their own or another segment's); callers never use it
+// (they select it away).
+template <int W, typename T>
+__device__ __forceinline__ T from_next(T v) {
+  if constexpr (W == 16) return dpp<DPP_ROW_SHL1>(v);
+  else return dpp<DPP_WAVE_SHL1>(v);
+}
+template <int W, typename T>
+__device__ __forceinline__ T from_prev(T v) {
+  if constexpr (W == 16) return dpp<DPP_ROW_SHR1>(v);
+  else return dpp<DPP_WAVE_SHR1>(v);
+}
+
+// value of lane-delta / lane+delta inside the segment (HIP semantics: lanes
+// whose source is outside the segment read their own value)
 template <int W, typename T>
 __device__ __forceinline__ T shup(T v, int delta) {
+  if (delta == 1) return from_prev<W>(v);
   return __shfl_up(v, static_cast<unsigned>(delta), W);
 }
 template <int W, typename T>
 __device__ __forceinline__ T shdn(T v, int delta) {
+  if (delta == 1) return from_next<W>(v);
   return __shfl_down(v, static_cast<unsigned>(delta), W);
 }
+
+// value of lane `src` of my segment; `src` is the same for every segment
 template <int W, typename T>
 __device__ __forceinline__ T shidx(T v, int src) {
-  return __shfl(v, src, W);
+  if constexpr (W == 64) {
+    if constexpr (sizeof(T) == 8) {
+      return mkdbl(__builtin_amdgcn_readlane(hi32(v), src), __builtin_amdgcn_readlane(lo32(v), src));
+    } else {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+    }
+  } else {
+    return __shfl(v, src, W);
+  }
 }
 
+// A wave-uniform value read from the launch arguments, pinned to scalar
+// registers (keeps the compiler from turning a select between two
+// coefficients into a per-lane load from a selected address).
+__device__ __forceinline__ double uni(const double& x) {
+  return mkdbl(__builtin_amdgcn_readfirstlane(hi32(x)), __builtin_amdgcn_readfirstlane(lo32(x)));
+}
+__device__ __forceinline__ float uni(const float& x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ int uni(const int& x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// A zero the compiler cannot see through.  Indexing a level's factors with it
+// keeps their scalar loads next to their use: otherwise every level's loads are
+// hoisted to the kernel entry and the scalar registers spill.
+__device__ __forceinline__ int opaque_zero() {
+  int z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  return z;
+}
+__device__ __forceinline__ void level_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// ---- the solve ----------------------------------------------------------------------
 // Solve in place.  `lane` is the lane index inside the segment (0..W-1).
 // Requires n <= W*R and R <= 2^nlev (top rows sit at position R-1 of a lane);
-// the host picks (W, R) accordingly.
-template <typename T, int R, int W>
+// the host picks (W, R) accordingly.  FULL: n == W*R, n a power of two.
+template <typename T, int R, int W, bool FULL = false>
 __device__ __forceinline__ void cr_solve_regs(T (&d)[R], const NtPlanT<T>& P, int nlev, int lane) {
   constexpr int LR = Log2<R>::value;
+  constexpr int LW = Log2<W>::value;
   const int row0 = lane * R;
+  const bool first_lane = (lane == 0);
+  const bool last_lane = (lane == W - 1);
 
   // ---- forward reduction, in-lane levels -------------------------------------
 #pragma unroll
   for (int s = 1; s <= LR; ++s) {
     const int h = 1 << (s - 1);
-    const NtLevel<T>& L = P.lv[s - 1];
-    const T nb = shdn<W>(d[h - 1], 1);      // row (lane+1)*R + h - 1
+    level_fence();
+    const NtLevel<T>& Lr = P.lv[s - 1 + opaque_zero()];
+    const T k1 = uni(Lr.k1), k2 = uni(Lr.k2), k1f = uni(Lr.k1f), kl = uni(Lr.kl);
+    const int m = FULL ? 0 : uni(Lr.m);
+    const T nb = from_next<W>(d[h - 1]);         // row (lane+1)*R + h - 1
 #pragma unroll
     for (int p = (1 << s) - 1; p < R; p += (1 << s)) {
-      const int jp = ((row0 + p + 1) >> (s - 1)) - 1;   // index at level s-1 (odd)
-      if (jp < L.m) {
-        const T left = d[p - h];
-        const T right = (p + h < R) ? d[p + h] : nb;
-        const T K1 = (jp == 1) ? L.k1f : ((jp == L.m - 1) ? L.kl : L.k1);
-        const T K2 = (jp == L.m - 1) ? T(0) : ((jp == L.m - 2) ? L.kl : L.k2);
-        d[p] = d[p] - K1 * left - K2 * right;
+      const T left = d[p - h];
+      const bool has_right_in_lane = (p + h < R);
+      const T right = has_right_in_lane ? d[p + h] : nb;
+      if constexpr (FULL) {
+        // only the first kept row of lane 0 and the last row of the last lane are special
+        T K1 = k1;
+        if (p == (1 << s) - 1) K1 = first_lane ? k1f : K1;
+        T nv;
+        if (p == R - 1) {
+          K1 = last_lane ? kl : K1;
+          nv = last_lane ? d[p] - K1 * left : d[p] - K1 * left - k2 * right;
+        } else {
+          nv = d[p] - K1 * left - k2 * right;
+        }
+        d[p] = nv;
+      } else {
+        const int jp = ((row0 + p + 1) >> (s - 1)) - 1;   // index at level s-1 (odd)
+        const bool valid = jp < m;
+        const bool is_last = (jp == m - 1);
+        const T K1 = (jp == 1) ? k1f : (is_last ? kl : k1);
+        const T K2 = (jp == m - 2) ? kl : k2;
+        const T with_r = d[p] - K1 * left - K2 * right;
+        const T no_r = d[p] - K1 * left;
+        d[p] = valid ? (is_last ? no_r : with_r) : d[p];
       }
     }
   }
 
   // ---- forward reduction, cross-lane levels ----------------------------------
-  constexpr int LW = Log2<W>::value;
   T v = d[R - 1];
 #pragma unroll
   for (int s = LR + 1; s <= LR + LW; ++s) {
-    if (s > nlev) break;
+    if (!FULL && s > nlev) break;
+    if (FULL && s > LR + LW - 1) break;          // n = W*R: nlev = LR + LW - 1
     const int sh = s - 1 - LR;
     const int dl = 1 << sh;
-    const NtLevel<T>& L = P.lv[s - 1];
+    level_fence();
+    const NtLevel<T>& Lr = P.lv[s - 1 + opaque_zero()];
+    const T k1 = uni(Lr.k1), k2 = uni(Lr.k2), k1f = uni(Lr.k1f), kl = uni(Lr.kl);
+    const int m = FULL ? 0 : uni(Lr.m);
     const T left = shup<W>(v, dl);
     const T right = shdn<W>(v, dl);
     const int jp = ((lane + 1) >> sh) - 1;
-    const bool kept = (((lane + 1) & ((dl << 1) - 1)) == 0) && (jp < L.m);
-    const T K1 = (jp == 1) ? L.k1f : ((jp == L.m - 1) ? L.kl : L.k1);
-    const T K2 = (jp == L.m - 1) ? T(0) : ((jp == L.m - 2) ? L.kl : L.k2);
-    const T nv = v - K1 * left - K2 * right;
-    v = kept ? nv : v;
+    const bool kept = (((lane + 1) & ((dl << 1) - 1)) == 0) && (FULL || jp < m);
+    const bool is_last = FULL ? last_lane : (jp == m - 1);
+    const T K1 = (jp == 1) ? k1f : (is_last ? kl : k1);
+    const T K2 = (!FULL && jp == m - 2) ? kl : k2;
+    const T with_r = v - K1 * left - K2 * right;
+    const T no_r = v - K1 * left;
+    v = kept ? (is_last ? no_r : with_r) : v;
   }
 
   // ---- top system (m = 2 or 3 equations at level nlev) ------------------------
   {
-    const NtTop<T>& Tp = P.top;
-    const int H = 1 << nlev;                 // row spacing at the top level
-    const int l0 = H / R - 1;                // lane holding top row 0
+    level_fence();
+    const NtTop<T>& Tp = (&P.top)[opaque_zero()];
+    const T t0 = uni(Tp.t[0]), t1 = uni(Tp.t[1]), t2 = uni(Tp.t[2]), t3 = uni(Tp.t[3]);
+    const int tm = FULL ? 2 : uni(Tp.m);
+    const int nl = FULL ? (LR + LW - 1) : nlev;
+    const int H = 1 << nl;                      // row spacing at the top level
+    const int l0 = H / R - 1;                   // lane holding top row 0
     const int l1 = 2 * H / R - 1;
     const T d0 = shidx<W>(v, l0);
     const T d1 = shidx<W>(v, l1);
-    if (Tp.m == 2) {
-      const T x0 = d0 * Tp.t[0] - d1 * Tp.t[1];
-      const T x1 = d1 * Tp.t[2] - d0 * Tp.t[3];
+    if (tm == 2) {
+      const T x0 = d0 * t0 - d1 * t1;
+      const T x1 = d1 * t2 - d0 * t3;
       v = (lane == l0) ? x0 : ((lane == l1) ? x1 : v);
     } else {
       const int l2 = 3 * H / R - 1;
       const T d2 = shidx<W>(v, l2 < W ? l2 : l1);
-      const T x1 = (d1 - Tp.t[0] * d0 - Tp.t[1] * d2) * Tp.t[2];
-      const T x0 = (d0 - Tp.t[3] * x1) * Tp.t[4];
-      const T x2 = (d2 - Tp.t[5] * x1) * Tp.t[6];
+      const T x1 = (d1 - t0 * d0 - t1 * d2) * t2;
+      const T x0 = (d0 - t3 * x1) * uni(Tp.t[4]);
+      const T x2 = (d2 - uni(Tp.t[5]) * x1) * uni(Tp.t[6]);
       v = (lane == l0) ? x0 : ((lane == l1) ? x1 : ((lane == l2) ? x2 : v));
     }
   }
@@ -116,38 +230,69 @@ __device__ __forceinline__ void cr_solve_regs(T (&d)[R], const NtPlanT<T>& P, in
   // ---- back substitution, cross-lane levels ----------------------------------
 #pragma unroll
   for (int s = LR + LW; s >= LR + 1; --s) {
-    if (s > nlev) continue;
+    if (!FULL && s > nlev) continue;
+    if (FULL && s > LR + LW - 1) continue;
     const int sh = s - 1 - LR;
     const int dl = 1 << sh;
-    const NtLevel<T>& L = P.lv[s - 1];
+    level_fence();
+    const NtLevel<T>& Lr = P.lv[s - 1 + opaque_zero()];
+    const T a = uni(Lr.a), c = uni(Lr.c), ib = uni(Lr.ib), cf = uni(Lr.cf), ibf = uni(Lr.ibf);
+    const T al = FULL ? T(0) : uni(Lr.al), ibl = FULL ? T(0) : uni(Lr.ibl);
+    const int m = FULL ? 0 : uni(Lr.m);
     const T left = shup<W>(v, dl);
     const T right = shdn<W>(v, dl);
     const int jp = ((lane + 1) >> sh) - 1;
-    const bool elim = (((lane + 1) & (dl - 1)) == 0) && ((jp & 1) == 0) && (jp < L.m);
-    const T A = (jp == 0) ? T(0) : ((jp == L.m - 1) ? L.al : L.a);
-    const T C = (jp == L.m - 1) ? T(0) : ((jp == 0) ? L.cf : L.c);
-    const T IB = (jp == 0) ? L.ibf : ((jp == L.m - 1) ? L.ibl : L.ib);
-    const T nv = (v - A * left - C * right) * IB;
+    const bool elim = (((lane + 1) & (dl - 1)) == 0) && ((jp & 1) == 0) && (FULL || jp < m);
+    const bool is_first = (jp == 0);
+    const bool is_last = !FULL && (jp == m - 1);
+    // interior x = (v - a*xl - c*xr) / b; row 0: (v - cF*xr) / bF; row m-1 (m odd): (v - aL*xl) / bL
+    const T A = is_last ? al : a;
+    const T C = is_first ? cf : c;
+    const T IB = is_first ? ibf : (is_last ? ibl : ib);
+    const T t_first = (v - C * right) * IB;
+    const T t_last = (v - A * left) * IB;
+    const T t_mid = (v - A * left - C * right) * IB;
+    const T nv = is_first ? t_first : (is_last ? t_last : t_mid);
     v = elim ? nv : v;
   }
   d[R - 1] = v;
 
   // ---- back substitution, in-lane levels --------------------------------------
-  const T prev = shup<W>(v, 1);              // solved row lane*R - 1 (lane 0: unused)
+  const T prev = from_prev<W>(v);              // solved row lane*R - 1 (lane 0: unused)
 #pragma unroll
   for (int s = LR; s >= 1; --s) {
     const int h = 1 << (s - 1);
-    const NtLevel<T>& L = P.lv[s - 1];
+    level_fence();
+    const NtLevel<T>& Lr = P.lv[s - 1 + opaque_zero()];
+    const T a = uni(Lr.a), c = uni(Lr.c), ib = uni(Lr.ib), cf = uni(Lr.cf), ibf = uni(Lr.ibf);
+    const T al = FULL ? T(0) : uni(Lr.al), ibl = FULL ? T(0) : uni(Lr.ibl);
+    const int m = FULL ? 0 : uni(Lr.m);
 #pragma unroll
     for (int p = h - 1; p < R; p += 2 * h) {
-      const int jp = ((row0 + p + 1) >> (s - 1)) - 1;   // even
-      if (jp < L.m) {
-        const T left = (p - h >= 0) ? d[p - h] : prev;
-        const T right = d[(p + h < R) ? p + h : p];
-        const T A = (jp == 0) ? T(0) : ((jp == L.m - 1) ? L.al : L.a);
-        const T C = (jp == L.m - 1) ? T(0) : ((jp == 0) ? L.cf : L.c);
-        const T IB = (jp == 0) ? L.ibf : ((jp == L.m - 1) ? L.ibl : L.ib);
-        d[p] = (d[p] - A * left - C * right) * IB;
+      const T left = (p - h >= 0) ? d[p - h] : prev;
+      const T right = d[(p + h < R) ? p + h : p];
+      if constexpr (FULL) {
+        // eliminated rows: row 0 of the system only at lane 0, p = h-1; never the last row
+        const T mid = (d[p] - a * left - c * right) * ib;
+        if (p == h - 1) {
+          const T fst = (d[p] - cf * right) * ibf;
+          d[p] = first_lane ? fst : mid;
+        } else {
+          d[p] = mid;
+        }
+      } else {
+        const int jp = ((row0 + p + 1) >> (s - 1)) - 1;   // even
+        const bool valid = jp < m;
+        const bool is_first = (jp == 0);
+        const bool is_last = (jp == m - 1);
+        const T A = is_last ? al : a;
+        const T C = is_first ? cf : c;
+        const T IB = is_first ? ibf : (is_last ? ibl : ib);
+        const T t_first = (d[p] - C * right) * IB;
+        const T t_last = (d[p] - A * left) * IB;
+        const T t_mid = (d[p] - A * left - C * right) * IB;
+        const T nv = is_first ? t_first : (is_last ? t_last : t_mid);
+        d[p] = valid ? nv : d[p];
       }
     }
   }

@@ -1,5 +1,5 @@
-// Launch-shape selection for the line kernel.  Each (input type, output type,
-// layout, producer, precision) combination is instantiated in its own
+// Launch-shape selection for the line kernels.  Each (input type, output
+// type, layout, producer, precision) combination is instantiated in its own
 // translation unit so the library builds in parallel.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -15,40 +15,63 @@ template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
 hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 
 // Launch-shape variants, chosen per (layout, ceil(log2 n)) by the tuning table
-// (nt_tune_set; defaults 0).  CONTIG: 0 LDS-staged tile of 4 lines; 1 direct
-// chunk loads, 4 waves per group; 2 direct, 1 wave per group.  STRIDED: 0 tile
-// of 16 lines (16 lanes per line); 1 tile of 32 lines.
+// (nt_tune_set; default 0).  Every variant computes the same values.
+//   CONTIG  0: one line per wavefront, wave-coalesced nontemporal loads and
+//              stores, wave-private LDS transpose (needs 16-byte aligned lines)
+//           1: one line per wavefront, each lane loads its own chunk directly
+//           2: workgroup tile of 4 lines staged through LDS (any n, alignment)
+//   STRIDED 0: tile of 16 adjacent lines, 16 lanes per line
+//           1: tile of 32 adjacent lines, 16 lanes per line
+// Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 constexpr int kNumVariants[2] = {3, 2};
 int tune_get(int layout, int n);
 void tune_set(int layout, int n, int variant);
 
-#define NT_GO(R_, W_, WAVES_, E_)                                                          \
+// n == W*R with n a power of two: the level structure is known at compile time
+inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) == 0; }
+
+#define NT_LAUNCH_(KERNEL, TS_, NT_)                                                       \
   do {                                                                                     \
-    constexpr int TS_ = (WAVES_) * (64 / (W_));                                            \
     const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
-    a.tiles = (nsys_ + TS_ - 1) / TS_;                                                     \
-    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    a.tiles = (nsys_ + (TS_)-1) / (TS_);                                                   \
     const int64_t grid_ = outer * a.tiles;                                                 \
     if (grid_ <= 0) return hipSuccess;                                                     \
     if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
-    hipLaunchKernelGGL((line_kernel<TIn, TOut, R_, W_, WAVES_, E_, LAYOUT, PROD, PREC>),   \
-                       dim3(static_cast<unsigned>(grid_)), dim3((WAVES_) * 64), 0, st, a); \
+    hipLaunchKernelGGL(KERNEL, dim3(static_cast<unsigned>(grid_)), dim3(NT_), 0, st, a);   \
     return hipGetLastError();                                                              \
   } while (0)
 
-// direct (LDS-free) contiguous-line kernel; one segment of W lanes per line
+// LDS-staged tile kernel (CONTIG variant 2, all STRIDED variants)
+#define NT_GO(R_, W_, WAVES_, E_)                                                          \
+  do {                                                                                     \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    if (is_full(n, W_, R_))                                                                \
+      NT_LAUNCH_((line_kernel<TIn, TOut, R_, W_, WAVES_, E_, LAYOUT, PROD, PREC, true>),   \
+                 (WAVES_) * (64 / (W_)), (WAVES_)*64);                                     \
+    NT_LAUNCH_((line_kernel<TIn, TOut, R_, W_, WAVES_, E_, LAYOUT, PROD, PREC, false>),    \
+               (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
+  } while (0)
+
+// direct chunk-load kernel (CONTIG variant 1)
 #define NT_GOD(R_, W_, WAVES_)                                                             \
   do {                                                                                     \
-    constexpr int TS_ = (WAVES_) * (64 / (W_));                                            \
-    const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
-    a.tiles = (nsys_ + TS_ - 1) / TS_;                                                     \
     if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
-    const int64_t grid_ = outer * a.tiles;                                                 \
-    if (grid_ <= 0) return hipSuccess;                                                     \
-    if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
-    hipLaunchKernelGGL((line_kernel_direct<TIn, TOut, R_, W_, WAVES_, PROD, PREC>),        \
-                       dim3(static_cast<unsigned>(grid_)), dim3((WAVES_) * 64), 0, st, a); \
-    return hipGetLastError();                                                              \
+    if (is_full(n, W_, R_))                                                                \
+      NT_LAUNCH_((line_kernel_direct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, true>),        \
+                 (WAVES_) * (64 / (W_)), (WAVES_)*64);                                     \
+    NT_LAUNCH_((line_kernel_direct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, false>),         \
+               (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
+  } while (0)
+
+// wave-coalesced streaming kernel (CONTIG variant 0)
+#define NT_GOW(R_, W_, WAVES_)                                                             \
+  do {                                                                                     \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    if (is_full(n, W_, R_))                                                                \
+      NT_LAUNCH_((line_kernel_wct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, true>),           \
+                 (WAVES_) * (64 / (W_)), (WAVES_)*64);                                     \
+    NT_LAUNCH_((line_kernel_wct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, false>),            \
+               (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
   } while (0)
 
 // 16-byte chunk accesses need 16-byte aligned line starts
@@ -63,35 +86,23 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
   const int n = a.plan.n;
   const int variant = tune_get(LAYOUT, n);
   if constexpr (LAYOUT == CONTIG) {
-   if (variant > 0 && n > 64 && direct_ok(a, sizeof(TIn), sizeof(TOut))) {
-    if (variant == 1) {
+    const int vin = 16 / static_cast<int>(sizeof(TIn)), vout = 16 / static_cast<int>(sizeof(TOut));
+    const bool aligned = n > 64 && direct_ok(a, sizeof(TIn), sizeof(TOut));
+    if (variant == 0 && aligned && a.pstride == n && n % vin == 0 && n % vout == 0) {
+      if (n <= 128) NT_GOW(2, 64, 4);
+      if (n <= 256) NT_GOW(4, 64, 4);
+      if (n <= 512) NT_GOW(8, 64, 4);
+      if (n <= 1024) NT_GOW(16, 64, 4);
+      if (n <= 2048) NT_GOW(32, 64, 2);
+    }
+    if (variant <= 1 && aligned) {
       if (n <= 128) NT_GOD(2, 64, 4);
       if (n <= 256) NT_GOD(4, 64, 4);
       if (n <= 512) NT_GOD(8, 64, 4);
       if (n <= 1024) NT_GOD(16, 64, 4);
       if (n <= 2048) NT_GOD(32, 64, 4);
-    } else {
-      if (n <= 128) NT_GOD(2, 64, 1);
-      if (n <= 256) NT_GOD(4, 64, 1);
-      if (n <= 512) NT_GOD(8, 64, 1);
-      if (n <= 1024) NT_GOD(16, 64, 1);
-      if (n <= 2048) NT_GOD(32, 64, 1);
     }
-   }
-  }
-  if constexpr (LAYOUT == STRIDED) {
-   if (variant == 1) {
-    // 32 adjacent lines per tile (256-byte FP64 rows)
-    if (n <= 16) NT_GO(1, 16, 8, 16);
-    if (n <= 32) NT_GO(2, 16, 8, 32);
-    if (n <= 64) NT_GO(4, 16, 8, 64);
-    if (n <= 128) NT_GO(8, 16, 8, 64);
-    if (n <= 256) NT_GO(16, 16, 8, 64);
-    if (n <= 512) NT_GO(32, 16, 8, 64);
-   }
-  }
-  if constexpr (LAYOUT == CONTIG) {
-    // one system per 64-lane wavefront (16 lanes for tiny systems), 4 waves
+    // LDS-staged workgroup tiles: one system per wavefront (16 lanes for tiny systems)
     if (n <= 16) NT_GO(1, 16, 4, 16);
     if (n <= 64) NT_GO(1, 64, 4, 64);
     if (n <= 128) NT_GO(2, 64, 4, 128);
@@ -100,7 +111,16 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     if (n <= 1024) NT_GO(16, 64, 4, 1024);
     if (n <= 2048) NT_GO(32, 64, 4, 2048);
   } else {
-    // 16 adjacent lines per tile (128-byte rows in FP64), streamed in blocks
+    if (variant == 1) {
+      // 32 adjacent lines per tile (256-byte FP64 rows)
+      if (n <= 16) NT_GO(1, 16, 8, 16);
+      if (n <= 32) NT_GO(2, 16, 8, 32);
+      if (n <= 64) NT_GO(4, 16, 8, 64);
+      if (n <= 128) NT_GO(8, 16, 8, 64);
+      if (n <= 256) NT_GO(16, 16, 8, 64);
+      if (n <= 512) NT_GO(32, 16, 8, 64);
+    }
+    // 16 adjacent lines per tile (128-byte FP64 rows), streamed in blocks
     if (n <= 16) NT_GO(1, 16, 4, 16);
     if (n <= 32) NT_GO(2, 16, 4, 32);
     if (n <= 64) NT_GO(4, 16, 4, 64);

@@ -99,18 +99,18 @@ __device__ __forceinline__ void compact_rhs_chunk(const double (&f)[R], double f
 
 // Solve A x = d for my chunk: FP64 CR, or FP32 CR + `sweeps` FP64-residual
 // refinement sweeps (x += A32^{-1} (d - A x), residual in FP64).
-template <int R, int W, int PREC>
+template <int R, int W, int PREC, bool FULL = false>
 __device__ __forceinline__ void solve_chunk(const double (&d)[R], double (&x)[R], const NtPlanK& P,
                                             int lane, int sweeps) {
   if (PREC == PREC_F64) {
 #pragma unroll
     for (int k = 0; k < R; ++k) x[k] = d[k];
-    cr_solve_regs<double, R, W>(x, P.d, P.nlev, lane);
+    cr_solve_regs<double, R, W, FULL>(x, P.d, P.nlev, lane);
   } else {
     float s[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) s[k] = static_cast<float>(d[k]);
-    cr_solve_regs<float, R, W>(s, P.f, P.nlev, lane);
+    cr_solve_regs<float, R, W, FULL>(s, P.f, P.nlev, lane);
 #pragma unroll
     for (int k = 0; k < R; ++k) x[k] = static_cast<double>(s[k]);
     for (int it = 0; it < sweeps; ++it) {
@@ -118,14 +118,15 @@ __device__ __forceinline__ void solve_chunk(const double (&d)[R], double (&x)[R]
       residual<R, W>(d, x, r, P, lane);
 #pragma unroll
       for (int k = 0; k < R; ++k) s[k] = static_cast<float>(r[k]);
-      cr_solve_regs<float, R, W>(s, P.f, P.nlev, lane);
+      cr_solve_regs<float, R, W, FULL>(s, P.f, P.nlev, lane);
 #pragma unroll
       for (int k = 0; k < R; ++k) x[k] += static_cast<double>(s[k]);
     }
   }
 }
 
-template <typename TIn, typename TOut, int R, int W, int WAVES, int E, int LAYOUT, int PROD, int PREC>
+template <typename TIn, typename TOut, int R, int W, int WAVES, int E, int LAYOUT, int PROD, int PREC,
+          bool FULL>
 __global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args) {
   constexpr int NT = WAVES * 64;
   constexpr int SPW = 64 / W;                 // systems (segments) per wave
@@ -226,7 +227,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args) {
     for (int k = 0; k < R; ++k) d[k] = f[k];
   }
   double x[R];
-  solve_chunk<R, W, PREC>(d, x, P, lane, args.sweeps);
+  solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
 
   // ---------------- store: blocks back through LDS ---------------------------
   for (int b = 0; b < nblk; ++b) {
@@ -319,7 +320,7 @@ __device__ __forceinline__ void store_chunk(T* __restrict__ line, int e0, int n,
 // together read the line's contiguous bytes, neighbour rows cross lanes by
 // shuffles, and there is no LDS and no barrier.  Each segment (W lanes) owns
 // one line, a workgroup TS = WAVES*64/W lines.
-template <typename TIn, typename TOut, int R, int W, int WAVES, int PROD, int PREC>
+template <typename TIn, typename TOut, int R, int W, int WAVES, int PROD, int PREC, bool FULL>
 __global__ void __launch_bounds__(WAVES * 64) line_kernel_direct(const LineArgs args) {
   constexpr int SPW = 64 / W;
   constexpr int TS = WAVES * SPW;
@@ -362,8 +363,130 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_direct(const LineArgs 
     for (int k = 0; k < R; ++k) d[k] = f[k];
   }
   double x[R];
-  solve_chunk<R, W, PREC>(d, x, P, lane, args.sweeps);
+  solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
   store_chunk<TOut, R>(oline, my_e0, n, x);
+}
+
+// ---- contiguous lines: wave-coalesced streaming loads + wave-private LDS transpose --
+// The lines of one wavefront are read and written with fully coalesced
+// nontemporal 16-byte accesses (instruction k of the wave covers bytes
+// 1024k .. 1024k+1023 of a line), and the wavefront's own LDS slice turns that
+// order into the chunked register layout of the solve and back.  No
+// workgroup barrier: each wavefront owns its lines and its LDS slice.
+typedef double nt_dbl2 __attribute__((ext_vector_type(2)));
+typedef float nt_flt4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename TIn, typename TOut, int R, int W, int WAVES, int PROD, int PREC, bool FULL>
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs args) {
+  constexpr int SPW = 64 / W;                 // lines per wavefront
+  constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
+  constexpr int CS = R + 2;                   // LDS doubles per chunk (2 pad: bank spread)
+  constexpr int LS = W * CS;                  // LDS doubles per line
+  static_assert(R >= 2, "chunk pairs need R >= 2");
+  __shared__ double lds_all[WAVES * SPW * LS];
+
+  const NtPlanK& P = args.plan;
+  const int n = P.n;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int l64 = tid & 63;
+  const int seg = l64 / W;
+  const int lane = l64 & (W - 1);
+  double* __restrict__ L = lds_all + wave * SPW * LS;
+  const int64_t nsys = args.phys - 2 * HALO;
+  const int64_t q0 = (static_cast<int64_t>(blockIdx.x) * WAVES + wave) * SPW;   // first line of the wave
+  if (q0 >= nsys) return;                     // the whole wavefront leaves together
+  const int nl = (nsys - q0 < SPW) ? static_cast<int>(nsys - q0) : SPW;
+  auto at = [&](int s, int e) -> int { return s * LS + (e / R) * CS + (e % R); };
+
+  // ---- coalesced streaming load of the wave's lines into LDS --------------------
+  for (int s = 0; s < nl; ++s) {
+    const TIn* __restrict__ src = static_cast<const TIn*>(args.in) + (q0 + s + HALO) * args.pstride;
+    if constexpr (sizeof(TIn) == 8) {
+      const nt_dbl2* v = reinterpret_cast<const nt_dbl2*>(src);
+      for (int j = l64; j < n / 2; j += 64) {
+        const nt_dbl2 t = __builtin_nontemporal_load(v + j);
+        L[at(s, 2 * j)] = t.x;
+        L[at(s, 2 * j + 1)] = t.y;
+      }
+    } else {
+      const nt_flt4* v = reinterpret_cast<const nt_flt4*>(src);
+      for (int j = l64; j < n / 4; j += 64) {
+        const nt_flt4 t = __builtin_nontemporal_load(v + j);
+        L[at(s, 4 * j)] = t.x;
+        L[at(s, 4 * j + 1)] = t.y;
+        L[at(s, 4 * j + 2)] = t.z;
+        L[at(s, 4 * j + 3)] = t.w;
+      }
+    }
+  }
+  wave_lds_sync();
+
+  // ---- my chunk, right-hand side, solve ---------------------------------------------
+  const int my_e0 = lane * R;
+  double f[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k] = (my_e0 + k < n) ? L[seg * LS + lane * CS + k] : 0.0;
+  double d[R];
+  if (PROD == P_COMPACT) {
+    double bnd[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) bnd[i] = L[at(seg, (i < 3) ? i : n + 2 - i)];
+    const double fl = shup<W>(f[R - 1], 1);
+    const double fr = shdn<W>(f[0], 1);
+    compact_rhs_chunk<R>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+  } else if (PROD == P_ADI) {
+    const TIn* __restrict__ line =
+        static_cast<const TIn*>(args.in) + (q0 + seg + HALO) * args.pstride;
+    double up[R], dn[R];
+    load_chunk<TIn, R>(line - args.pstride, my_e0, n, up);
+    load_chunk<TIn, R>(line + args.pstride, my_e0, n, dn);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int e = my_e0 + k;
+      d[k] = (e == 0 || e >= n - 1) ? f[k] : args.s1 * (up[k] + dn[k]) + args.s2 * f[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) d[k] = f[k];
+  }
+  double x[R];
+  solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+    if (my_e0 + k < n) L[seg * LS + lane * CS + k] = x[k];
+  wave_lds_sync();
+
+  // ---- coalesced streaming store ------------------------------------------------
+  for (int s = 0; s < nl; ++s) {
+    TOut* __restrict__ dst = static_cast<TOut*>(args.out) + (q0 + s + HALO) * args.pstride;
+    if constexpr (sizeof(TOut) == 8) {
+      nt_dbl2* v = reinterpret_cast<nt_dbl2*>(dst);
+      for (int j = l64; j < n / 2; j += 64) {
+        nt_dbl2 t;
+        t.x = L[at(s, 2 * j)];
+        t.y = L[at(s, 2 * j + 1)];
+        __builtin_nontemporal_store(t, v + j);
+      }
+    } else {
+      nt_flt4* v = reinterpret_cast<nt_flt4*>(dst);
+      for (int j = l64; j < n / 4; j += 64) {
+        nt_flt4 t;
+        t.x = static_cast<float>(L[at(s, 4 * j)]);
+        t.y = static_cast<float>(L[at(s, 4 * j + 1)]);
+        t.z = static_cast<float>(L[at(s, 4 * j + 2)]);
+        t.w = static_cast<float>(L[at(s, 4 * j + 3)]);
+        __builtin_nontemporal_store(t, v + j);
+      }
+    }
+  }
 }
 
 }  // namespace nt
