@@ -51,9 +51,10 @@ constexpr int DPP_ROW_SHR1 = 0x111;
 constexpr int DPP_WAVE_SHL1 = 0x130;
 constexpr int DPP_WAVE_SHR1 = 0x138;
 
+// lanes without a source lane read 0 (bound_ctrl)
 template <int CTRL>
 __device__ __forceinline__ int dpp32(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
 }
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
@@ -65,8 +66,8 @@ __device__ __forceinline__ double dpp(double v) {
 }
 
 // value of lane+1 / lane-1.  Lanes whose neighbour is outside the segment get
-// an unspecified value (their own or another segment's); callers never use it
-// (they select it away).
+// an unspecified value (0 or another segment's); callers never use it (they
+// select it away).
 template <int W, typename T>
 __device__ __forceinline__ T from_next(T v) {
   if constexpr (W == 16) return dpp<DPP_ROW_SHL1>(v);
@@ -78,17 +79,26 @@ __device__ __forceinline__ T from_prev(T v) {
   else return dpp<DPP_WAVE_SHR1>(v);
 }
 
-// value of lane-delta / lane+delta inside the segment (HIP semantics: lanes
-// whose source is outside the segment read their own value)
+// ds_bpermute of both halves from wavefront lane `src` (any lane, wraps mod 64)
+__device__ __forceinline__ double bperm(double v, int src) {
+  const int a = (src & 63) << 2;
+  return mkdbl(__builtin_amdgcn_ds_bpermute(a, hi32(v)), __builtin_amdgcn_ds_bpermute(a, lo32(v)));
+}
+__device__ __forceinline__ float bperm(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute((src & 63) << 2, __float_as_int(v)));
+}
+
+// value of lane-delta / lane+delta.  Lanes whose source is outside the
+// segment read an unspecified value, which callers select away.
 template <int W, typename T>
 __device__ __forceinline__ T shup(T v, int delta) {
   if (delta == 1) return from_prev<W>(v);
-  return __shfl_up(v, static_cast<unsigned>(delta), W);
+  return bperm(v, static_cast<int>(__lane_id()) - delta);
 }
 template <int W, typename T>
 __device__ __forceinline__ T shdn(T v, int delta) {
   if (delta == 1) return from_next<W>(v);
-  return __shfl_down(v, static_cast<unsigned>(delta), W);
+  return bperm(v, static_cast<int>(__lane_id()) + delta);
 }
 
 // value of lane `src` of my segment; `src` is the same for every segment
@@ -101,7 +111,7 @@ __device__ __forceinline__ T shidx(T v, int src) {
       return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
     }
   } else {
-    return __shfl(v, src, W);
+    return bperm(v, static_cast<int>(__lane_id() & ~(W - 1)) + src);
   }
 }
 

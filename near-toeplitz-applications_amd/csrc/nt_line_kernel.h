@@ -80,8 +80,9 @@ __device__ __forceinline__ void residual(const double (&d)[R], const double (&x)
 
 // Compact RHS (compact/rhs.py:21-36) of my chunk from the chunk values f, the
 // neighbour rows fl (row my_e0-1) and fr (row my_e0+R) and the six boundary
-// values bnd = f[0], f[1], f[2], f[n-1], f[n-2], f[n-3].
-template <int R>
+// values bnd = f[0], f[1], f[2], f[n-1], f[n-2], f[n-3].  FULL (n == W*R):
+// only the first row of lane 0 and the last row of the last lane are closures.
+template <int R, bool FULL = false>
 __device__ __forceinline__ void compact_rhs_chunk(const double (&f)[R], double fl, double fr,
                                                   const double (&bnd)[6], int my_e0, int n,
                                                   double s1, double s2, double (&d)[R]) {
@@ -93,7 +94,14 @@ __device__ __forceinline__ void compact_rhs_chunk(const double (&f)[R], double f
     const double left = (k > 0) ? f[k - 1] : fl;
     const double right = (k + 1 < R) ? f[k + 1] : fr;
     const double in_v = s1 * (right - left);
-    d[k] = (e == 0) ? b0 : ((e == n - 1) ? bn : ((e < n) ? in_v : 0.0));
+    if constexpr (FULL) {
+      double v = in_v;
+      if (k == 0) v = (e == 0) ? b0 : v;
+      if (k == R - 1) v = (e == n - 1) ? bn : v;
+      d[k] = v;
+    } else {
+      d[k] = (e == 0) ? b0 : ((e == n - 1) ? bn : ((e < n) ? in_v : 0.0));
+    }
   }
 }
 
@@ -137,7 +145,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args) {
   constexpr int EP = (E / R) * (R + CP) + 2;  // LDS stride between staged lines
   static_assert(E % R == 0, "a chunk must not straddle blocks");
 
-  __shared__ double lds[ROWS * EP];
+  __shared__ __attribute__((aligned(16))) double lds[ROWS * EP];
 
   const NtPlanK& P = args.plan;
   const int n = P.n;
@@ -221,7 +229,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args) {
   if (PROD == P_COMPACT) {
     const double fl = shup<W>(f[R - 1], 1);      // row my_e0 - 1
     const double fr = shdn<W>(f[0], 1);          // row my_e0 + R
-    compact_rhs_chunk<R>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+    compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
   } else {
 #pragma unroll
     for (int k = 0; k < R; ++k) d[k] = f[k];
@@ -348,7 +356,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_direct(const LineArgs 
     for (int i = 0; i < 6; ++i) bnd[i] = static_cast<double>(line[(i < 3) ? i : n + 2 - i]);
     const double fl = shup<W>(f[R - 1], 1);
     const double fr = shdn<W>(f[0], 1);
-    compact_rhs_chunk<R>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+    compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
   } else if (PROD == P_ADI) {
     double up[R], dn[R];
     load_chunk<TIn, R>(line - args.pstride, my_e0, n, up);
@@ -389,7 +397,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
   constexpr int CS = R + 2;                   // LDS doubles per chunk (2 pad: bank spread)
   constexpr int LS = W * CS;                  // LDS doubles per line
   static_assert(R >= 2, "chunk pairs need R >= 2");
-  __shared__ double lds_all[WAVES * SPW * LS];
+  __shared__ __attribute__((aligned(16))) double lds_all[WAVES * SPW * LS];
 
   const NtPlanK& P = args.plan;
   const int n = P.n;
@@ -406,18 +414,23 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
   auto at = [&](int s, int e) -> int { return s * LS + (e / R) * CS + (e % R); };
 
   // ---- coalesced streaming load of the wave's lines into LDS --------------------
+  // (FULL: n == W*R, so each loop runs a compile-time number of times)
+  const int nv2 = FULL ? W * R / 2 : n / 2;
+  const int nv4 = FULL ? W * R / 4 : n / 4;
   for (int s = 0; s < nl; ++s) {
     const TIn* __restrict__ src = static_cast<const TIn*>(args.in) + (q0 + s + HALO) * args.pstride;
     if constexpr (sizeof(TIn) == 8) {
       const nt_dbl2* v = reinterpret_cast<const nt_dbl2*>(src);
-      for (int j = l64; j < n / 2; j += 64) {
+#pragma unroll
+      for (int j = l64; j < nv2; j += 64) {
         const nt_dbl2 t = __builtin_nontemporal_load(v + j);
         L[at(s, 2 * j)] = t.x;
         L[at(s, 2 * j + 1)] = t.y;
       }
     } else {
       const nt_flt4* v = reinterpret_cast<const nt_flt4*>(src);
-      for (int j = l64; j < n / 4; j += 64) {
+#pragma unroll
+      for (int j = l64; j < nv4; j += 64) {
         const nt_flt4 t = __builtin_nontemporal_load(v + j);
         L[at(s, 4 * j)] = t.x;
         L[at(s, 4 * j + 1)] = t.y;
@@ -431,8 +444,18 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
   // ---- my chunk, right-hand side, solve ---------------------------------------------
   const int my_e0 = lane * R;
   double f[R];
+  double* __restrict__ mine = L + seg * LS + lane * CS;     // 16-byte aligned chunk
+  if (FULL || my_e0 + R <= n) {
 #pragma unroll
-  for (int k = 0; k < R; ++k) f[k] = (my_e0 + k < n) ? L[seg * LS + lane * CS + k] : 0.0;
+    for (int k = 0; k < R; k += 2) {
+      const nt_dbl2 t = *reinterpret_cast<const nt_dbl2*>(mine + k);
+      f[k] = t.x;
+      f[k + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) f[k] = (my_e0 + k < n) ? mine[k] : 0.0;
+  }
   double d[R];
   if (PROD == P_COMPACT) {
     double bnd[6];
@@ -440,7 +463,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
     for (int i = 0; i < 6; ++i) bnd[i] = L[at(seg, (i < 3) ? i : n + 2 - i)];
     const double fl = shup<W>(f[R - 1], 1);
     const double fr = shdn<W>(f[0], 1);
-    compact_rhs_chunk<R>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+    compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
   } else if (PROD == P_ADI) {
     const TIn* __restrict__ line =
         static_cast<const TIn*>(args.in) + (q0 + seg + HALO) * args.pstride;
@@ -459,9 +482,19 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
   double x[R];
   solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
   wave_lds_sync();
+  if (FULL || my_e0 + R <= n) {
 #pragma unroll
-  for (int k = 0; k < R; ++k)
-    if (my_e0 + k < n) L[seg * LS + lane * CS + k] = x[k];
+    for (int k = 0; k < R; k += 2) {
+      nt_dbl2 t;
+      t.x = x[k];
+      t.y = x[k + 1];
+      *reinterpret_cast<nt_dbl2*>(mine + k) = t;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      if (my_e0 + k < n) mine[k] = x[k];
+  }
   wave_lds_sync();
 
   // ---- coalesced streaming store ------------------------------------------------
@@ -469,7 +502,8 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
     TOut* __restrict__ dst = static_cast<TOut*>(args.out) + (q0 + s + HALO) * args.pstride;
     if constexpr (sizeof(TOut) == 8) {
       nt_dbl2* v = reinterpret_cast<nt_dbl2*>(dst);
-      for (int j = l64; j < n / 2; j += 64) {
+#pragma unroll
+      for (int j = l64; j < nv2; j += 64) {
         nt_dbl2 t;
         t.x = L[at(s, 2 * j)];
         t.y = L[at(s, 2 * j + 1)];
@@ -477,7 +511,8 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
       }
     } else {
       nt_flt4* v = reinterpret_cast<nt_flt4*>(dst);
-      for (int j = l64; j < n / 4; j += 64) {
+#pragma unroll
+      for (int j = l64; j < nv4; j += 64) {
         nt_flt4 t;
         t.x = static_cast<float>(L[at(s, 4 * j)]);
         t.y = static_cast<float>(L[at(s, 4 * j + 1)]);
