@@ -26,6 +26,7 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "compact_ddx_512_fp64_traffic.json")
 
 
 def log(msg):
@@ -130,6 +131,33 @@ def adi_case(N, r, nsteps, reps):
             "note": "32 MiB field stays in the 256 MB Infinity Cache: effective, not HBM, GB/s"}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy's readings)."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            return json.load(fh)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def pmc_probe():
+    """Three headline launches, then three copies of the same 1 GiB field: the
+    program profiled by the FETCH_SIZE / WRITE_SIZE passes."""
+    import torch
+    from ntamd import COMPACT_COEFFS, Plan, core
+    N = 512
+    f = torch.randn((N, N, N), dtype=torch.float64, device="cuda")
+    out = torch.empty_like(f)
+    plan = Plan(N, COMPACT_COEFFS)
+    for _ in range(3):
+        core.compact_derivative(plan, f, out, 1.0 / (N - 1), 2)
+    for _ in range(3):
+        out.copy_(f)
+    torch.cuda.synchronize()
+    print("pmc probe done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +165,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="headline only, no extras/CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run the short program profiled for HBM traffic and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -149,6 +179,10 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+
+    if args.pmc_probe:
+        pmc_probe()
+        return
 
     N = 512
     log(f"[bench] headline: compact d/dx {N}^3 FP64, steps={args.steps} warmup={args.warmup}")
@@ -198,7 +232,8 @@ def main():
                    "n": N, "systems": N * N, "coeffs": [1, 2, 0.25, 1, 0.25, 2, 1],
                    "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": head["gbps"], "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": head["hbm_frac"], "traffic": None,
+                     "unit": "GB/s", "frac": head["hbm_frac"], "traffic": measured_traffic(),
+                     "traffic_source": "profiles/compact_ddx_512_fp64.md (rocprofv3 --pmc)",
                      "alg_bytes_per_launch": head["alg_bytes"],
                      "kernel_mean_s": head["mean_s"], "kernel_median_s": head["median_s"]},
         "cpu_baseline": cpu,

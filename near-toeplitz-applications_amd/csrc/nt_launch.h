@@ -20,8 +20,8 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //              stores, wave-private LDS transpose (needs 16-byte aligned lines)
 //           1: one line per wavefront, each lane loads its own chunk directly
 //           2: workgroup tile of 4 lines staged through LDS (any n, alignment)
-//   STRIDED 0: tile of 16 adjacent lines, 16 lanes per line
-//           1: tile of 32 adjacent lines, 16 lanes per line
+//   STRIDED 0: tile of 16 adjacent lines, 64 lanes per line (1024 threads)
+//           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads)
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 constexpr int kNumVariants[2] = {3, 2};
 int tune_get(int layout, int n);
@@ -74,6 +74,17 @@ inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) ==
                (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
   } while (0)
 
+// strided-line tile kernel (STRIDED variants)
+#define NT_GOS(R_, W_, WAVES_, E_)                                                         \
+  do {                                                                                     \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    if (is_full(n, W_, R_))                                                                \
+      NT_LAUNCH_((line_kernel_strided<TIn, TOut, R_, W_, WAVES_, E_, PROD, PREC, true>),   \
+                 (WAVES_) * (64 / (W_)), (WAVES_)*64);                                     \
+    NT_LAUNCH_((line_kernel_strided<TIn, TOut, R_, W_, WAVES_, E_, PROD, PREC, false>),    \
+               (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
+  } while (0)
+
 // 16-byte chunk accesses need 16-byte aligned line starts
 inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
   const uintptr_t i = reinterpret_cast<uintptr_t>(a.in), o = reinterpret_cast<uintptr_t>(a.out);
@@ -112,23 +123,21 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     if (n <= 2048) NT_GO(32, 64, 4, 2048);
   } else {
     if (variant == 1) {
-      // 32 adjacent lines per tile (256-byte FP64 rows)
-      if (n <= 16) NT_GO(1, 16, 8, 16);
-      if (n <= 32) NT_GO(2, 16, 8, 32);
-      if (n <= 64) NT_GO(4, 16, 8, 64);
-      if (n <= 128) NT_GO(8, 16, 8, 64);
-      if (n <= 256) NT_GO(16, 16, 8, 64);
-      if (n <= 512) NT_GO(32, 16, 8, 64);
+      // 32 adjacent lines per tile (256-byte FP64 rows), 32 lanes per line
+      if (n <= 32) NT_GOS(1, 32, 16, 32);
+      if (n <= 64) NT_GOS(2, 32, 16, 64);
+      if (n <= 128) NT_GOS(4, 32, 16, 128);
+      if (n <= 256) NT_GOS(8, 32, 16, 128);
+      if (n <= 512) NT_GOS(16, 32, 16, 128);
+      if (n <= 1024) NT_GOS(32, 32, 16, 128);
     }
-    // 16 adjacent lines per tile (128-byte FP64 rows), streamed in blocks
-    if (n <= 16) NT_GO(1, 16, 4, 16);
-    if (n <= 32) NT_GO(2, 16, 4, 32);
-    if (n <= 64) NT_GO(4, 16, 4, 64);
-    if (n <= 128) NT_GO(8, 16, 4, 128);
-    if (n <= 256) NT_GO(16, 16, 4, 128);
-    if (n <= 512) NT_GO(32, 16, 4, 128);
-    if (n <= 1024) NT_GO(16, 64, 16, 128);
-    if (n <= 2048) NT_GO(32, 64, 16, 128);
+    // 16 adjacent lines per tile (128-byte FP64 rows), 64 lanes per line
+    if (n <= 64) NT_GOS(1, 64, 16, 64);
+    if (n <= 128) NT_GOS(2, 64, 16, 128);
+    if (n <= 256) NT_GOS(4, 64, 16, 128);
+    if (n <= 512) NT_GOS(8, 64, 16, 128);
+    if (n <= 1024) NT_GOS(16, 64, 16, 128);
+    if (n <= 2048) NT_GOS(32, 64, 16, 128);
   }
   return hipErrorInvalidValue;
 }

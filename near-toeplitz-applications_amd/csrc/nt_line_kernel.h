@@ -524,4 +524,176 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
   }
 }
 
+// ---- strided lines (y/z): workgroup tile of TS adjacent lines ------------------------
+// The tile is TS lines that are adjacent in memory (unit stride between
+// lines, estride between elements).  Blocks of E elements of all staged
+// lines go through LDS: a block row of the tile is TS+2*HALO contiguous values,
+// read with 16-byte nontemporal loads when aligned; each lane then copies its
+// chunk.  Lines are solved by segments of W lanes, so the chunk is R = n/W
+// rows (small for W = 64 or 32).  One workgroup barrier separates loading and
+// copying each block; blocks alternate between two LDS buffers.
+template <typename TIn, typename TOut, int R, int W, int WAVES, int E, int PROD, int PREC, bool FULL>
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs args) {
+  constexpr int NT = WAVES * 64;
+  constexpr int SPW = 64 / W;
+  constexpr int TS = WAVES * SPW;             // lines per tile
+  constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
+  constexpr int ROWS = TS + 2 * HALO;         // staged lines
+  constexpr int CP = (R >= 4) ? 2 : 0;
+  constexpr int EP = (E / R) * (R + CP) + 2;  // LDS stride between staged lines (doubles)
+  constexpr int BUF = ROWS * EP;
+  static_assert(E % R == 0, "a chunk must not straddle blocks");
+  __shared__ __attribute__((aligned(16))) double lds[2 * BUF];
+
+  const NtPlanK& P = args.plan;
+  const int n = FULL ? W * R : P.n;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int seg = (tid & 63) / W;
+  const int lane = tid & (W - 1);
+  const int ql = wave * SPW + seg;            // tile-local line
+
+  const int64_t bid = blockIdx.x;
+  const int64_t o = bid / args.tiles;
+  const int64_t p0 = (bid - o * args.tiles) * TS;   // first line of the tile
+  const int64_t nsys = args.phys - 2 * HALO;
+  const bool active = (p0 + ql) < nsys;
+  const int64_t base = o * args.ostride + p0;        // pstride == 1 for strided lines
+  const TIn* __restrict__ in = static_cast<const TIn*>(args.in) + base;
+  TOut* __restrict__ out = static_cast<TOut*>(args.out) + base;
+  const int64_t rows_valid64 = args.phys - p0;
+  const int rows_valid = rows_valid64 < ROWS ? static_cast<int>(rows_valid64) : ROWS;
+  // 16-byte row loads: whole tile in range, aligned, no halo
+  constexpr int VIN = 16 / sizeof(TIn);
+  const bool vec_in = (HALO == 0) && (rows_valid == ROWS) && (ROWS % VIN == 0) &&
+                      ((reinterpret_cast<uintptr_t>(in) % 16) == 0) &&
+                      ((args.estride * sizeof(TIn)) % 16 == 0);
+  constexpr int VOUT = 16 / sizeof(TOut);
+  const bool vec_out = (HALO == 0) && (p0 + TS <= nsys) && (TS % VOUT == 0) &&
+                       ((reinterpret_cast<uintptr_t>(out) % 16) == 0) &&
+                       ((args.estride * sizeof(TOut)) % 16 == 0);
+
+  const int my_e0 = lane * R;
+  const int my_blk = my_e0 / E;
+  const int nblk = (n + E - 1) / E;
+  auto at = [&](int buf, int row, int el) -> int { return buf * BUF + row * EP + el + (el / R) * CP; };
+
+  double f[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k] = 0.0;
+  double bnd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+
+  for (int b = 0; b < nblk; ++b) {
+    const int e0 = b * E;
+    const int buf = b & 1;
+    const int ecount = (n - e0 < E) ? n - e0 : E;
+    if (vec_in) {
+      constexpr int VPR = ROWS / VIN;          // vectors per block row
+      for (int idx = tid; idx < VPR * ecount; idx += NT) {
+        const int vr = idx % VPR, el = idx / VPR;
+        const TIn* src = in + static_cast<int64_t>(e0 + el) * args.estride + vr * VIN;
+        if constexpr (sizeof(TIn) == 8) {
+          const nt_dbl2 t = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(src));
+          lds[at(buf, vr * 2, el)] = t.x;
+          lds[at(buf, vr * 2 + 1, el)] = t.y;
+        } else {
+          const nt_flt4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_flt4*>(src));
+          lds[at(buf, vr * 4, el)] = t.x;
+          lds[at(buf, vr * 4 + 1, el)] = t.y;
+          lds[at(buf, vr * 4 + 2, el)] = t.z;
+          lds[at(buf, vr * 4 + 3, el)] = t.w;
+        }
+      }
+    } else {
+      for (int idx = tid; idx < ROWS * ecount; idx += NT) {
+        const int r = idx % ROWS, el = idx / ROWS;
+        if (r < rows_valid)
+          lds[at(buf, r, el)] = static_cast<double>(in[r + static_cast<int64_t>(e0 + el) * args.estride]);
+      }
+    }
+    __syncthreads();
+    if (my_blk == b && active) {
+      const int el0 = my_e0 - e0;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int e = my_e0 + k;
+        if (FULL || e < n) {
+          const double c = lds[at(buf, ql + HALO, el0 + k)];
+          if (PROD == P_ADI) {
+            const double up = lds[at(buf, ql, el0 + k)];
+            const double dn = lds[at(buf, ql + 2, el0 + k)];
+            f[k] = (e == 0 || e == n - 1) ? c : args.s1 * (up + dn) + args.s2 * c;
+          } else {
+            f[k] = c;
+          }
+        }
+      }
+    }
+    if (PROD == P_COMPACT && active) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const int e = (i < 3) ? i : n + 2 - i;
+        if (e >= e0 && e < e0 + E) bnd[i] = lds[at(buf, ql + HALO, e - e0)];
+      }
+    }
+    // the next block writes the other buffer; the one after that reuses this
+    // buffer only after the next iteration's barrier
+  }
+
+  double d[R];
+  if (PROD == P_COMPACT) {
+    const double fl = shup<W>(f[R - 1], 1);
+    const double fr = shdn<W>(f[0], 1);
+    compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) d[k] = f[k];
+  }
+  double x[R];
+  solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
+
+  for (int b = 0; b < nblk; ++b) {
+    const int e0 = b * E;
+    const int buf = b & 1;
+    const int ecount = (n - e0 < E) ? n - e0 : E;
+    // block 0 follows the load phase; later blocks reuse a buffer whose last
+    // readers all passed the previous block's barrier
+    if (b == 0) __syncthreads();
+    if (my_blk == b && active) {
+      const int el0 = my_e0 - e0;
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (FULL || my_e0 + k < n) lds[at(buf, ql + HALO, el0 + k)] = x[k];
+    }
+    __syncthreads();
+    if (vec_out) {
+      constexpr int VPR = TS / VOUT;
+      for (int idx = tid; idx < VPR * ecount; idx += NT) {
+        const int vr = idx % VPR, el = idx / VPR;
+        TOut* dst = out + static_cast<int64_t>(e0 + el) * args.estride + vr * VOUT;
+        if constexpr (sizeof(TOut) == 8) {
+          nt_dbl2 t;
+          t.x = lds[at(buf, vr * 2, el)];
+          t.y = lds[at(buf, vr * 2 + 1, el)];
+          __builtin_nontemporal_store(t, reinterpret_cast<nt_dbl2*>(dst));
+        } else {
+          nt_flt4 t;
+          t.x = static_cast<float>(lds[at(buf, vr * 4, el)]);
+          t.y = static_cast<float>(lds[at(buf, vr * 4 + 1, el)]);
+          t.z = static_cast<float>(lds[at(buf, vr * 4 + 2, el)]);
+          t.w = static_cast<float>(lds[at(buf, vr * 4 + 3, el)]);
+          __builtin_nontemporal_store(t, reinterpret_cast<nt_flt4*>(dst));
+        }
+      }
+    } else {
+      for (int idx = tid; idx < TS * ecount; idx += NT) {
+        const int r = idx % TS, el = idx / TS;
+        if (p0 + r < nsys)
+          out[(r + HALO) + static_cast<int64_t>(e0 + el) * args.estride] =
+              static_cast<TOut>(lds[at(buf, r + HALO, el)]);
+      }
+    }
+  }
+}
+
 }  // namespace nt
