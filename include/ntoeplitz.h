@@ -145,6 +145,35 @@ int nt_part_plan_secondary(const nt_part_plan* plan, int p, double* xU, double* 
 int nt_part_solve_lines(const nt_part_plan* plan, int producer, const void* in, void* out,
                         int64_t lines, double p1, double p2, void* stream);
 
+/* ---- mixed-size persistent batch --------------------------------------------
+ * No reference counterpart (the reference launches one grid per field and
+ * direction, run-compact.py:57-90).  Many fields of different extents,
+ * directions, storage types and precisions in one persistent launch that
+ * pulls tiles of 16 lines from a device work counter.  Lines of 16..512 points.
+ * The library allocates nothing: nt_batch_pack writes a binary image into a
+ * caller-provided host buffer (nt_batch_image_bytes long); the caller copies it
+ * to a 256-byte aligned device buffer, zeroes a device unsigned counter and
+ * calls nt_batch_launch on the stream of the copy.  Each field's result does
+ * not depend on the other jobs in the launch. */
+typedef struct nt_batch_job {
+  const void* in;     /* field (producer 1), or the systems to solve (producer 0; == out) */
+  void* out;          /* FP64 result */
+  int64_t nz, ny, nx; /* C-contiguous extents */
+  double dx;          /* grid spacing along axis (producer 1) */
+  int axis;           /* 0 = z, 1 = y, 2 = x */
+  int producer;       /* 0 = solve in place, 1 = compact derivative */
+  int in_dtype;       /* NT_F64 or NT_F32 (producer 1) */
+  int plan_slot;      /* index into the plans array of nt_batch_pack */
+  int precision;      /* NT_PREC_F64 or NT_PREC_F32_REFINE */
+  int sweeps;
+} nt_batch_job;
+size_t nt_batch_image_bytes(int njobs, int nplans);
+int nt_batch_pack(const nt_batch_job* jobs, int njobs, const nt_plan* const* plans, int nplans,
+                  void* host_image, size_t bytes, int64_t* total_tiles);
+/* grid <= 0: two workgroups per compute unit */
+int nt_batch_launch(const void* dev_image, int njobs, int nplans, int64_t total_tiles,
+                    unsigned int* dev_counter, int grid, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

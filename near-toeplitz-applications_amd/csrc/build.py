@@ -30,6 +30,7 @@ SOURCES = [
     "nt_k_adi_contig.hip",
     "nt_k_adi_strided.hip",
     "nt_part.hip",
+    "nt_batch.hip",
 ]
 
 ARCH = os.environ.get("NT_OFFLOAD_ARCH", "gfx950")

@@ -1,0 +1,280 @@
+// Mixed-size persistent batch: many fields of different shapes, derivative
+// directions, precisions and storage types solved by one launch.
+//
+// The host packs a binary image (job table + level tables of the plans the
+// jobs use) into a caller-provided host buffer; the caller copies it to the
+// device and zeroes a work counter; one persistent launch then pulls tiles of
+// 16 lines from the counter until every job is done.  A tile is 16 lines
+// adjacent in memory (16 rows of an x-line field, or 16 adjacent y/z lines),
+// one wavefront per line, staged through LDS; each wavefront runs the same
+// register CR as the other kernels (general-n path, up to 512 points).
+// Per-line arithmetic does not depend on which other jobs share the launch, so
+// a field gives the same bits alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "../../include/ntoeplitz.h"
+#include "nt_error.h"
+#include "nt_line_kernel.h"
+#include "nt_plan.h"
+
+namespace nt {
+
+constexpr int BATCH_MAXN = 512;
+constexpr int BATCH_R = 8;                 // 64 lanes x 8 rows
+constexpr int BATCH_TS = 16;               // lines per tile = wavefronts per workgroup
+constexpr int BATCH_EP = BATCH_MAXN + 2;   // LDS stride between lines (doubles)
+
+struct BatchJob {
+  const void* in;
+  void* out;
+  int64_t pstride, estride, ostride;  // line geometry (as LineArgs)
+  int64_t phys;                       // lines per slab
+  int64_t tiles;                      // tiles per slab
+  int64_t tile0;                      // first global tile of this job
+  int64_t ntiles;                     // tiles of this job (slabs x tiles)
+  double s1, s2;                      // compact scalars 3/(4h), 1/(2h)
+  int32_t plan;                       // index into the plan table
+  int32_t layout;                     // CONTIG / STRIDED
+  int32_t prod;                       // P_IDENT / P_COMPACT
+  int32_t in_f32;                     // input stored as FP32
+  int32_t prec;                       // PREC_F64 / PREC_F32R
+  int32_t sweeps;
+};
+
+struct BatchArgs {
+  const BatchJob* jobs;
+  const NtPlanK* plans;
+  unsigned int* counter;
+  int64_t total_tiles;
+  int32_t njobs;
+  int32_t pad_;
+};
+
+__device__ __forceinline__ int find_job(const BatchJob* jobs, int njobs, int64_t t) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {   // last job with tile0 <= t
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(BATCH_TS * 64) batch_kernel(const BatchArgs args) {
+  __shared__ __attribute__((aligned(16))) double lds[BATCH_TS * BATCH_EP];
+  __shared__ int64_t s_tile;
+  const int tid = threadIdx.x;
+  const int ql = tid >> 6;          // my line within the tile
+  const int lane = tid & 63;
+  constexpr int R = BATCH_R;
+  constexpr int NT = BATCH_TS * 64;
+
+  for (;;) {
+    if (tid == 0) s_tile = static_cast<int64_t>(atomicAdd(args.counter, 1u));
+    __syncthreads();
+    // the tile index is the same for the whole workgroup: keep it (and the
+    // job and plan addresses derived from it) in scalar registers
+    const int64_t t = static_cast<int64_t>(static_cast<unsigned>(uni(static_cast<int>(s_tile))));
+    if (t >= args.total_tiles) break;            // every workgroup reaches this
+    const int j = find_job(args.jobs, args.njobs, t);
+    const BatchJob& J = args.jobs[j];
+    const NtPlanK& P = args.plans[J.plan];
+    const int n = P.n;
+    const int64_t local = t - J.tile0;
+    const int64_t o = local / J.tiles;
+    const int64_t p0 = (local - o * J.tiles) * BATCH_TS;
+    const int64_t base = o * J.ostride + p0 * J.pstride;
+    const int64_t rows64 = J.phys - p0;
+    const int rows = rows64 < BATCH_TS ? static_cast<int>(rows64) : BATCH_TS;
+
+    // ---- stage the tile (lines x n) through LDS, coalesced along memory ----------
+    if (J.layout == CONTIG) {
+      for (int idx = tid; idx < rows * n; idx += NT) {
+        const int r = idx / n, e = idx - r * n;
+        const int64_t g = base + r * J.pstride + e;
+        lds[r * BATCH_EP + e] = J.in_f32 ? static_cast<double>(static_cast<const float*>(J.in)[g])
+                                         : static_cast<const double*>(J.in)[g];
+      }
+    } else {
+      for (int idx = tid; idx < BATCH_TS * n; idx += NT) {
+        const int r = idx % BATCH_TS, e = idx / BATCH_TS;
+        if (r < rows) {
+          const int64_t g = base + r + static_cast<int64_t>(e) * J.estride;
+          lds[r * BATCH_EP + e] = J.in_f32 ? static_cast<double>(static_cast<const float*>(J.in)[g])
+                                           : static_cast<const double*>(J.in)[g];
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- my line: chunk, RHS, solve ---------------------------------------------------
+    const bool active = ql < rows;
+    const int my_e0 = lane * R;
+    double f[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) f[k] = (active && my_e0 + k < n) ? lds[ql * BATCH_EP + my_e0 + k] : 0.0;
+    double d[R];
+    if (J.prod == P_COMPACT) {
+      double bnd[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) bnd[i] = lds[ql * BATCH_EP + ((i < 3) ? i : n + 2 - i)];
+      const double fl = shup<64>(f[R - 1], 1);
+      const double fr = shdn<64>(f[0], 1);
+      compact_rhs_chunk<R, false>(f, fl, fr, bnd, my_e0, n, J.s1, J.s2, d);
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; ++k) d[k] = f[k];
+    }
+    double x[R];
+    if (J.prec == PREC_F64) solve_chunk<R, 64, PREC_F64, false>(d, x, P, lane, J.sweeps);
+    else solve_chunk<R, 64, PREC_F32R, false>(d, x, P, lane, J.sweeps);
+    __syncthreads();   // every line finished reading the tile
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (my_e0 + k < n) lds[ql * BATCH_EP + my_e0 + k] = x[k];
+    }
+    __syncthreads();
+
+    // ---- store --------------------------------------------------------------------
+    double* __restrict__ out = static_cast<double*>(J.out);
+    if (J.layout == CONTIG) {
+      for (int idx = tid; idx < rows * n; idx += NT) {
+        const int r = idx / n, e = idx - r * n;
+        out[base + r * J.pstride + e] = lds[r * BATCH_EP + e];
+      }
+    } else {
+      for (int idx = tid; idx < BATCH_TS * n; idx += NT) {
+        const int r = idx % BATCH_TS, e = idx / BATCH_TS;
+        if (r < rows) out[base + r + static_cast<int64_t>(e) * J.estride] = lds[r * BATCH_EP + e];
+      }
+    }
+    __syncthreads();   // LDS and s_tile are reused by the next tile
+  }
+}
+
+}  // namespace nt
+
+namespace {
+
+size_t image_bytes(int njobs, int nplans) {
+  const size_t jb = sizeof(nt::BatchJob) * static_cast<size_t>(njobs);
+  const size_t jb_al = (jb + 255) & ~static_cast<size_t>(255);
+  return jb_al + sizeof(NtPlanK) * static_cast<size_t>(nplans);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t nt_batch_image_bytes(int njobs, int nplans) {
+  if (njobs < 0 || nplans < 0) return 0;
+  return image_bytes(njobs, nplans);
+}
+
+int nt_batch_pack(const nt_batch_job* jobs, int njobs, const nt_plan* const* plans, int nplans,
+                  void* host_image, size_t bytes, int64_t* total_tiles) {
+  if (!jobs || !plans || !host_image || !total_tiles || njobs <= 0 || nplans <= 0)
+    return nt::set_error(NT_EINVAL, "%s", "nt_batch_pack: null or empty argument");
+  if (bytes < image_bytes(njobs, nplans))
+    return nt::set_error(NT_EINVAL, "nt_batch_pack: image needs %zu bytes", image_bytes(njobs, nplans));
+  std::memset(host_image, 0, image_bytes(njobs, nplans));
+  auto* J = static_cast<nt::BatchJob*>(host_image);
+  const size_t jb = sizeof(nt::BatchJob) * static_cast<size_t>(njobs);
+  auto* Pk = reinterpret_cast<NtPlanK*>(static_cast<char*>(host_image) + ((jb + 255) & ~static_cast<size_t>(255)));
+  for (int i = 0; i < nplans; ++i) {
+    if (!plans[i]) return nt::set_error(NT_EINVAL, "nt_batch_pack: plan %d is null", i);
+    Pk[i] = plans[i]->k;
+  }
+  int64_t tile0 = 0;
+  for (int i = 0; i < njobs; ++i) {
+    const nt_batch_job& q = jobs[i];
+    if (!q.in || !q.out) return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: null buffer", i);
+    if (q.plan_slot < 0 || q.plan_slot >= nplans)
+      return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: plan slot %d", i, q.plan_slot);
+    if (q.axis < 0 || q.axis > 2) return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: axis", i);
+    if (q.nz <= 0 || q.ny <= 0 || q.nx <= 0) return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: extents", i);
+    if (q.producer != 0 && q.producer != 1)
+      return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: producer must be 0 (solve) or 1 (compact)", i);
+    if (q.in_dtype != NT_F64 && q.in_dtype != NT_F32)
+      return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: in_dtype", i);
+    if (q.producer == 0 && (q.in != q.out || q.in_dtype != NT_F64))
+      return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: solves are in place on FP64 fields", i);
+    if (q.producer == 1 && q.in == q.out)
+      return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: derivative input and output alias", i);
+    if (q.precision != NT_PREC_F64 && q.precision != NT_PREC_F32_REFINE)
+      return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: precision", i);
+    if (q.sweeps < 0 || q.sweeps > 8) return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: sweeps", i);
+    const int64_t n = (q.axis == 2) ? q.nx : (q.axis == 1 ? q.ny : q.nz);
+    if (n != plans[q.plan_slot]->n)
+      return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: line length %lld != plan n", i, (long long)n);
+    // 64 lanes x 8 rows: the solve needs at least 3 reductions (n >= 16)
+    if (n > nt::BATCH_MAXN || n < 16)
+      return nt::set_error(NT_EUNSUPPORTED, "nt_batch_pack: job %d: batched lines need 16 <= n <= 512", i);
+    if (q.producer == 1 && !(std::isfinite(q.dx) && q.dx != 0.0))
+      return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: dx", i);
+    nt::BatchJob& b = J[i];
+    b.in = q.in;
+    b.out = q.out;
+    int64_t outer;
+    if (q.axis == 2) {
+      b.layout = nt::CONTIG; outer = 1; b.phys = q.nz * q.ny; b.pstride = q.nx; b.estride = 1; b.ostride = 0;
+    } else if (q.axis == 1) {
+      b.layout = nt::STRIDED; outer = q.nz; b.phys = q.nx; b.pstride = 1; b.estride = q.nx; b.ostride = q.ny * q.nx;
+    } else {
+      b.layout = nt::STRIDED; outer = 1; b.phys = q.ny * q.nx; b.pstride = 1; b.estride = q.ny * q.nx; b.ostride = 0;
+    }
+    b.tiles = (b.phys + nt::BATCH_TS - 1) / nt::BATCH_TS;
+    b.ntiles = outer * b.tiles;
+    b.tile0 = tile0;
+    tile0 += b.ntiles;
+    b.s1 = (q.producer == 1) ? 3.0 / (4.0 * q.dx) : 0.0;
+    b.s2 = (q.producer == 1) ? 1.0 / (2.0 * q.dx) : 0.0;
+    b.plan = q.plan_slot;
+    b.prod = (q.producer == 1) ? nt::P_COMPACT : nt::P_IDENT;
+    b.in_f32 = (q.in_dtype == NT_F32);
+    b.prec = (q.precision == NT_PREC_F64) ? nt::PREC_F64 : nt::PREC_F32R;
+    b.sweeps = q.sweeps;
+  }
+  *total_tiles = tile0;
+  return nt::set_error(NT_OK, "%s", "");
+}
+
+int nt_batch_launch(const void* dev_image, int njobs, int nplans, int64_t total_tiles,
+                    unsigned int* dev_counter, int grid, void* stream) {
+  if (!dev_image || !dev_counter || njobs <= 0 || nplans <= 0)
+    return nt::set_error(NT_EINVAL, "%s", "nt_batch_launch: null or empty argument");
+  if (total_tiles <= 0) return nt::set_error(NT_OK, "%s", "");
+  if (total_tiles >= 0xffffffffLL) return nt::set_error(NT_EINVAL, "%s", "nt_batch_launch: too many tiles");
+  if (reinterpret_cast<uintptr_t>(dev_image) % 256 != 0)
+    return nt::set_error(NT_EINVAL, "%s", "nt_batch_launch: image must be 256-byte aligned");
+  int g = grid;
+  if (g <= 0) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      hipDeviceProp_t prop;
+      if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    g = 2 * cus;   // two 1024-thread workgroups per CU
+  }
+  if (g > total_tiles) g = static_cast<int>(total_tiles);
+  nt::BatchArgs a;
+  a.jobs = static_cast<const nt::BatchJob*>(dev_image);
+  const size_t jb = sizeof(nt::BatchJob) * static_cast<size_t>(njobs);
+  a.plans = reinterpret_cast<const NtPlanK*>(static_cast<const char*>(dev_image) +
+                                             ((jb + 255) & ~static_cast<size_t>(255)));
+  a.counter = dev_counter;
+  a.total_tiles = total_tiles;
+  a.njobs = njobs;
+  a.pad_ = 0;
+  hipLaunchKernelGGL(nt::batch_kernel, dim3(g), dim3(nt::BATCH_TS * 64), 0,
+                     static_cast<hipStream_t>(stream), a);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return nt::set_error(NT_EHIP, "nt_batch_launch: %s", hipGetErrorString(e));
+  return nt::set_error(NT_OK, "%s", "");
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+"""Mixed-size persistent batches: many fields, one launch (nt_batch_* in the C ABI).
+
+    batch = MixedBatch([dict(f=f1, out=o1, axis=2, dx=h1),
+                        dict(f=f2, out=o2, axis=0, dx=h2, precision="fp32_refine", sweeps=1),
+                        ...])
+    batch.run()            # one persistent launch on the current stream
+
+The job table and the level tables of the distinct plans are packed once on the
+host by the library, copied to one device buffer, and reused by every run(); a
+run zeroes the work counter with a kernel (graph-capturable) and launches.
+Fields must stay alive, at the same addresses, for the life of the batch.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .core import Plan, device_ptr, precision_code, stream_handle
+
+__all__ = ["NtBatchJob", "MixedBatch"]
+
+COMPACT = (1.0, 2.0, 0.25, 1.0, 0.25, 2.0, 1.0)
+
+
+class NtBatchJob(ctypes.Structure):
+    """Mirror of ``nt_batch_job`` (include/ntoeplitz.h)."""
+    _fields_ = [("in_", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("nz", ctypes.c_int64), ("ny", ctypes.c_int64), ("nx", ctypes.c_int64),
+                ("dx", ctypes.c_double), ("axis", ctypes.c_int), ("producer", ctypes.c_int),
+                ("in_dtype", ctypes.c_int), ("plan_slot", ctypes.c_int),
+                ("precision", ctypes.c_int), ("sweeps", ctypes.c_int)]
+
+
+class MixedBatch:
+    """A fixed set of compact-derivative (or in-place solve) jobs run by one launch.
+
+    Each job is a dict: ``f`` (3-D FP64 or FP32 device tensor), ``out`` (FP64,
+    same shape), ``axis`` (0 z, 1 y, 2 x), ``dx``; optional ``precision``
+    ("fp64" | "fp32_refine"), ``sweeps``, ``coeffs`` (default: the compact
+    matrix), ``solve=True`` for an in-place solve of ``out`` (then ``f`` is
+    ignored).  Lines must have 16..512 points.
+    """
+
+    def __init__(self, jobs, grid=0):
+        if not jobs:
+            raise ValueError("empty batch")
+        lib = _lib.load()
+        self._keep = []
+        plans, slot = [], {}
+        arr = (NtBatchJob * len(jobs))()
+        dev = None
+        for i, j in enumerate(jobs):
+            out = j["out"]
+            solve = bool(j.get("solve", False))
+            f = out if solve else j["f"]
+            fp = device_ptr(f, "f", dtypes=(torch.float64, torch.float32))
+            op = device_ptr(out, "out")
+            if f.dim() != 3 or tuple(out.shape) != tuple(f.shape):
+                raise _lib.NtInvalidArgument(_lib.NT_EINVAL, f"job {i}: f/out must be 3-D, same shape")
+            dev = out.device if dev is None else dev
+            axis = int(j["axis"])
+            n = int(f.shape[axis])
+            coeffs = tuple(float(c) for c in j.get("coeffs", COMPACT))
+            key = (n, coeffs)
+            if key not in slot:
+                slot[key] = len(plans)
+                plans.append(Plan(n, coeffs))
+            a = arr[i]
+            a.in_, a.out = fp.value, op.value
+            a.nz, a.ny, a.nx = (int(s) for s in f.shape)
+            a.dx = float(j.get("dx", 1.0))
+            a.axis = axis
+            a.producer = 0 if solve else 1
+            a.in_dtype = _lib.NT_F32 if f.dtype == torch.float32 else _lib.NT_F64
+            a.plan_slot = slot[key]
+            a.precision = precision_code(j.get("precision", "fp64"))
+            a.sweeps = int(j.get("sweeps", 1))
+            self._keep += [f, out]
+        self.plans = plans
+        pv = (ctypes.c_void_p * len(plans))(*[p.handle.value for p in plans])
+        nbytes = lib.nt_batch_image_bytes(len(jobs), len(plans))
+        host = np.zeros(nbytes, dtype=np.uint8)
+        total = ctypes.c_int64()
+        _lib.check(lib.nt_batch_pack(arr, len(jobs), pv, len(plans), host.ctypes.data_as(ctypes.c_void_p),
+                                     ctypes.c_size_t(nbytes), ctypes.byref(total)))
+        self.total_tiles = total.value
+        self.njobs, self.nplans = len(jobs), len(plans)
+        self.image = torch.from_numpy(host).to(dev)              # device copy of the image
+        if self.image.data_ptr() % 256:
+            raise RuntimeError("device image is not 256-byte aligned")
+        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.grid = int(grid)
+
+    def run(self):
+        self.counter.zero_()
+        _lib.call("nt_batch_launch", ctypes.c_void_p(self.image.data_ptr()), self.njobs, self.nplans,
+                  ctypes.c_int64(self.total_tiles), ctypes.c_void_p(self.counter.data_ptr()),
+                  self.grid, stream_handle())
