@@ -158,6 +158,92 @@ def pmc_probe():
     print("pmc probe done", flush=True)
 
 
+def config4_case(N=1024, reps=5):
+    """1024^3 field stored in FP32 (4 GiB): d/dx, d/dy, d/dz with the FP32 solve +
+    one FP64-residual sweep, against plain FP64 (FP64-stored field, FP64 solve).
+    Errors are relative to the FP64 solve of the same (FP32-stored) values."""
+    import torch
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.workloads import compact_field_device
+    f32 = compact_field_device(N, seed=0, dtype=torch.float32)
+    f64 = f32.double()
+    out_r = torch.empty((N, N, N), dtype=torch.float64, device=f32.device)
+    out_64 = torch.empty_like(out_r)
+    plan = Plan(N, COMPACT_COEFFS)
+    h = 1.0 / (N - 1)
+    res = {}
+    for axis, name in ((2, "ddx"), (1, "ddy"), (0, "ddz")):
+        tr = time_launches(lambda: core.compact_derivative(plan, f32, out_r, h, axis, "fp32_refine", 1),
+                           reps, 2)
+        t64 = time_launches(lambda: core.compact_derivative(plan, f64, out_64, h, axis, "fp64", 1), reps, 2)
+        ref64 = torch.empty_like(out_r)
+        core.compact_derivative(plan, f32, ref64, h, axis, "fp64", 1)    # FP64 solve of the same values
+        err = ((out_r - ref64).abs().max() / ref64.abs().max()).item()
+        res[name] = {"fp32_refine_s": tr["mean_s"], "fp64_s": t64["mean_s"],
+                     "speedup_vs_fp64": t64["mean_s"] / tr["mean_s"], "max_rel_err_vs_fp64": err,
+                     "fp32_refine_gbps": N ** 3 * 12 / tr["mean_s"] / 1e9,
+                     "fp64_gbps": N ** 3 * 16 / t64["mean_s"] / 1e9}
+        log(f"[bench] 1024^3 {name}: fp32+ref {tr['mean_s'] * 1e3:.3f} ms, fp64 {t64['mean_s'] * 1e3:.3f} ms, "
+            f"x{res[name]['speedup_vs_fp64']:.2f}, err {err:.2e}")
+        del ref64
+    del f32, f64, out_r, out_64
+    torch.cuda.empty_cache()
+    return res
+
+
+def mixed_batch_case(count=64, seed=1, reps=3):
+    """64 grids, per-axis sizes uniform in {64, 96, ..., 512} (seeded), random
+    derivative direction each, FP32 solve + one sweep, all in one persistent launch.
+    Per-grid max relative error against the plain FP64 kernel."""
+    import numpy as np
+    import torch
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.batch import MixedBatch
+    from ntamd.workloads import compact_field_device
+    rng = np.random.default_rng(seed)
+    sizes = np.arange(64, 513, 32)
+    jobs, pts = [], 0
+    for i in range(count):
+        shape = tuple(int(s) for s in rng.choice(sizes, size=3))
+        axis = int(rng.integers(0, 3))
+        f = compact_field_device(shape, seed=100 + i)
+        jobs.append(dict(f=f, out=torch.empty_like(f), axis=axis, dx=1.0 / (shape[axis] - 1),
+                         precision="fp32_refine", sweeps=1))
+        pts += int(np.prod(shape))
+    b = MixedBatch(jobs)
+    t = time_launches(b.run, reps, 1)
+    errs = []
+    for j in jobs:
+        ref = torch.empty_like(j["out"])
+        n = j["f"].shape[j["axis"]]
+        core.compact_derivative(Plan(n, COMPACT_COEFFS), j["f"], ref, j["dx"], j["axis"], "fp64", 1)
+        errs.append(((j["out"] - ref).abs().max() / ref.abs().max()).item())
+    res = {"grids": count, "points": pts, "launch_s": t["mean_s"], "pts_per_s": pts / t["mean_s"],
+           "gbps": pts * 16 / t["mean_s"] / 1e9, "max_rel_err_vs_fp64_per_grid_max": max(errs),
+           "max_rel_err_vs_fp64_per_grid": errs}
+    log(f"[bench] mixed batch: {count} grids, {pts / 1e6:.0f} M pts, {t['mean_s'] * 1e3:.3f} ms, "
+        f"{res['gbps']:.0f} GB/s, worst grid err {max(errs):.2e}")
+    del jobs, b
+    torch.cuda.empty_cache()
+    return res
+
+
+def serving_case(count=200, rate=500.0, seed=2):
+    """Poisson arrivals (rate req/s): compact derivatives of random-phase trig
+    fields (per-axis sizes U[32,512], random axis) and ADI runs (side U[256,2048],
+    1-20 steps, r U[0.1,2]), FP64 or FP32+refinement each; continuous batching vs
+    one-at-a-time plain FP64."""
+    from ntamd.serve import Server, make_requests
+    out = {"rate_rps": rate, "requests": count}
+    for mode in ("continuous", "sequential"):
+        reqs = make_requests(count, rate, seed=seed)
+        stats = Server().run(reqs, mode=mode)
+        out[mode] = stats
+        log(f"[bench] serving {mode}: p50 {stats['p50_ms']:.2f} ms, p99 {stats['p99_ms']:.2f} ms, "
+            f"{stats['throughput_rps']:.0f} req/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,11 +260,12 @@ def main():
         sys.exit(2)
 
     import torch
-    from ntamd import _lib
+    from ntamd import _lib, autotune
     _lib.load()
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    tuned = autotune.apply()          # launch shapes chosen by the autotuner, if cached
 
     if args.pmc_probe:
         pmc_probe()
@@ -207,6 +294,9 @@ def main():
         a = adi_case(2048, 0.5, 200, 3)
         extras["adi_2048_200steps_graph"] = a
         log(f"[bench] ADI 2048^2: {a['ms_per_step']:.3f} ms/step")
+        extras["fp32_storage_1024"] = config4_case()
+        extras["mixed_batch_64"] = mixed_batch_case()
+        extras["serving"] = serving_case()
 
     cpu = None
     if not args.no_cpu:
@@ -239,6 +329,7 @@ def main():
         "cpu_baseline": cpu,
         "wall_s_timed_region": head["wall_s"],
         "extras": extras,
+        "tuning": tuned,
     }
     print(json.dumps(rec), flush=True)
 

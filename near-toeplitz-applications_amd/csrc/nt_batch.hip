@@ -24,7 +24,7 @@ namespace nt {
 
 constexpr int BATCH_MAXN = 512;
 constexpr int BATCH_R = 8;                 // 64 lanes x 8 rows
-constexpr int BATCH_TS = 16;               // lines per tile = wavefronts per workgroup
+constexpr int BATCH_TS = 8;                // lines per tile = wavefronts per workgroup
 constexpr int BATCH_EP = BATCH_MAXN + 2;   // LDS stride between lines (doubles)
 
 struct BatchJob {
@@ -53,71 +53,75 @@ struct BatchArgs {
   int32_t pad_;
 };
 
-__device__ __forceinline__ int find_job(const BatchJob* jobs, int njobs, int64_t t) {
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {   // last job with tile0 <= t
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].tile0 <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 __global__ void __launch_bounds__(BATCH_TS * 64) batch_kernel(const BatchArgs args) {
   __shared__ __attribute__((aligned(16))) double lds[BATCH_TS * BATCH_EP];
+  __shared__ __attribute__((aligned(16))) NtPlanK s_plan;   // level table of the current job
   __shared__ int64_t s_tile;
   const int tid = threadIdx.x;
   const int ql = tid >> 6;          // my line within the tile
   const int lane = tid & 63;
   constexpr int R = BATCH_R;
   constexpr int NT = BATCH_TS * 64;
+  int j = 0;                        // current job: tiles come in increasing order
+  int plan_in_lds = -1;
 
   for (;;) {
     if (tid == 0) s_tile = static_cast<int64_t>(atomicAdd(args.counter, 1u));
     __syncthreads();
     // the tile index is the same for the whole workgroup: keep it (and the
-    // job and plan addresses derived from it) in scalar registers
+    // job index derived from it) in scalar registers
     const int64_t t = static_cast<int64_t>(static_cast<unsigned>(uni(static_cast<int>(s_tile))));
     if (t >= args.total_tiles) break;            // every workgroup reaches this
-    const int j = find_job(args.jobs, args.njobs, t);
+    while (j + 1 < args.njobs && args.jobs[j + 1].tile0 <= t) ++j;
     const BatchJob& J = args.jobs[j];
-    const NtPlanK& P = args.plans[J.plan];
-    const int n = P.n;
+    const int plan = uni(J.plan);
+    if (plan != plan_in_lds) {                   // copy the level table into LDS
+      const int* src = reinterpret_cast<const int*>(args.plans + plan);
+      int* dst = reinterpret_cast<int*>(&s_plan);
+      for (int i = tid; i < static_cast<int>(sizeof(NtPlanK) / 4); i += NT) dst[i] = src[i];
+      plan_in_lds = plan;
+    }
     const int64_t local = t - J.tile0;
     const int64_t o = local / J.tiles;
     const int64_t p0 = (local - o * J.tiles) * BATCH_TS;
     const int64_t base = o * J.ostride + p0 * J.pstride;
     const int64_t rows64 = J.phys - p0;
     const int rows = rows64 < BATCH_TS ? static_cast<int>(rows64) : BATCH_TS;
+    const int layout = uni(J.layout), in_f32 = uni(J.in_f32), prod = uni(J.prod), prec = uni(J.prec);
+    const int n = uni(args.plans[plan].n);
+    const int64_t pstride = J.pstride, estride = J.estride;
+    const void* in = J.in;
+    double* __restrict__ out = static_cast<double*>(J.out);
 
     // ---- stage the tile (lines x n) through LDS, coalesced along memory ----------
-    if (J.layout == CONTIG) {
-      for (int idx = tid; idx < rows * n; idx += NT) {
-        const int r = idx / n, e = idx - r * n;
-        const int64_t g = base + r * J.pstride + e;
-        lds[r * BATCH_EP + e] = J.in_f32 ? static_cast<double>(static_cast<const float*>(J.in)[g])
-                                         : static_cast<const double*>(J.in)[g];
+    if (layout == CONTIG) {
+      if (ql < rows) {                           // wavefront ql loads its own line
+        const int64_t g0 = base + ql * pstride;
+        for (int e = lane; e < n; e += 64)
+          lds[ql * BATCH_EP + e] = in_f32 ? static_cast<double>(static_cast<const float*>(in)[g0 + e])
+                                          : static_cast<const double*>(in)[g0 + e];
       }
     } else {
       for (int idx = tid; idx < BATCH_TS * n; idx += NT) {
         const int r = idx % BATCH_TS, e = idx / BATCH_TS;
         if (r < rows) {
-          const int64_t g = base + r + static_cast<int64_t>(e) * J.estride;
-          lds[r * BATCH_EP + e] = J.in_f32 ? static_cast<double>(static_cast<const float*>(J.in)[g])
-                                           : static_cast<const double*>(J.in)[g];
+          const int64_t g = base + r + static_cast<int64_t>(e) * estride;
+          lds[r * BATCH_EP + e] = in_f32 ? static_cast<double>(static_cast<const float*>(in)[g])
+                                         : static_cast<const double*>(in)[g];
         }
       }
     }
     __syncthreads();
 
     // ---- my line: chunk, RHS, solve ---------------------------------------------------
+    const NtPlanK& P = s_plan;
     const bool active = ql < rows;
     const int my_e0 = lane * R;
     double f[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) f[k] = (active && my_e0 + k < n) ? lds[ql * BATCH_EP + my_e0 + k] : 0.0;
     double d[R];
-    if (J.prod == P_COMPACT) {
+    if (prod == P_COMPACT) {
       double bnd[6];
 #pragma unroll
       for (int i = 0; i < 6; ++i) bnd[i] = lds[ql * BATCH_EP + ((i < 3) ? i : n + 2 - i)];
@@ -129,7 +133,7 @@ __global__ void __launch_bounds__(BATCH_TS * 64) batch_kernel(const BatchArgs ar
       for (int k = 0; k < R; ++k) d[k] = f[k];
     }
     double x[R];
-    if (J.prec == PREC_F64) solve_chunk<R, 64, PREC_F64, false>(d, x, P, lane, J.sweeps);
+    if (prec == PREC_F64) solve_chunk<R, 64, PREC_F64, false>(d, x, P, lane, J.sweeps);
     else solve_chunk<R, 64, PREC_F32R, false>(d, x, P, lane, J.sweeps);
     __syncthreads();   // every line finished reading the tile
     if (active) {
@@ -140,19 +144,18 @@ __global__ void __launch_bounds__(BATCH_TS * 64) batch_kernel(const BatchArgs ar
     __syncthreads();
 
     // ---- store --------------------------------------------------------------------
-    double* __restrict__ out = static_cast<double*>(J.out);
-    if (J.layout == CONTIG) {
-      for (int idx = tid; idx < rows * n; idx += NT) {
-        const int r = idx / n, e = idx - r * n;
-        out[base + r * J.pstride + e] = lds[r * BATCH_EP + e];
+    if (layout == CONTIG) {
+      if (ql < rows) {
+        const int64_t g0 = base + ql * pstride;
+        for (int e = lane; e < n; e += 64) out[g0 + e] = lds[ql * BATCH_EP + e];
       }
     } else {
       for (int idx = tid; idx < BATCH_TS * n; idx += NT) {
         const int r = idx % BATCH_TS, e = idx / BATCH_TS;
-        if (r < rows) out[base + r + static_cast<int64_t>(e) * J.estride] = lds[r * BATCH_EP + e];
+        if (r < rows) out[base + r + static_cast<int64_t>(e) * estride] = lds[r * BATCH_EP + e];
       }
     }
-    __syncthreads();   // LDS and s_tile are reused by the next tile
+    __syncthreads();   // LDS, the plan copy and s_tile are reused by the next tile
   }
 }
 
@@ -258,7 +261,7 @@ int nt_batch_launch(const void* dev_image, int njobs, int nplans, int64_t total_
       hipDeviceProp_t prop;
       if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
-    g = 2 * cus;   // two 1024-thread workgroups per CU
+    g = 4 * cus;   // several workgroups per CU keep loads and solves overlapping
   }
   if (g > total_tiles) g = static_cast<int>(total_tiles);
   nt::BatchArgs a;

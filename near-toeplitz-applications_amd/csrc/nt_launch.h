@@ -21,7 +21,8 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //           1: one line per wavefront, each lane loads its own chunk directly
 //           2: workgroup tile of 4 lines staged through LDS (any n, alignment)
 //   STRIDED 0: tile of 16 adjacent lines, 64 lanes per line (1024 threads)
-//           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads)
+//           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads);
+//              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads)
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 constexpr int kNumVariants[2] = {3, 2};
 int tune_get(int layout, int n);
@@ -129,7 +130,9 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
       if (n <= 128) NT_GOS(4, 32, 16, 128);
       if (n <= 256) NT_GOS(8, 32, 16, 128);
       if (n <= 512) NT_GOS(16, 32, 16, 128);
-      if (n <= 1024) NT_GOS(32, 32, 16, 128);
+      // long lines: 8 adjacent lines, 64 lanes each, 512 threads (room for 32-row chunks)
+      if (n <= 1024) NT_GOS(16, 64, 8, 512);
+      if (n <= 2048) NT_GOS(32, 64, 8, 512);
     }
     // 16 adjacent lines per tile (128-byte FP64 rows), 64 lanes per line
     if (n <= 64) NT_GOS(1, 64, 16, 64);

@@ -43,6 +43,15 @@ struct LineArgs {
   int32_t pad_;
 };
 
+// The launch arguments read in place from the kernel-argument segment (the
+// struct is the kernel's only argument, at offset 0).  Accessing them through
+// this pointer keeps every level-table read a scalar load from constant
+// memory; using the by-value parameter with a run-time level index makes the
+// compiler copy the whole table to scratch.
+__device__ __forceinline__ const LineArgs& kernel_args() {
+  return *(const LineArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+}
+
 template <typename T>
 __device__ __forceinline__ double ld_as_double(const T* p) {
   return static_cast<double>(*p);
@@ -135,7 +144,9 @@ __device__ __forceinline__ void solve_chunk(const double (&d)[R], double (&x)[R]
 
 template <typename TIn, typename TOut, int R, int W, int WAVES, int E, int LAYOUT, int PROD, int PREC,
           bool FULL>
-__global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args) {
+__global__ void __launch_bounds__(WAVES * 64) line_kernel(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
   constexpr int NT = WAVES * 64;
   constexpr int SPW = 64 / W;                 // systems (segments) per wave
   constexpr int TS = WAVES * SPW;             // systems per tile
@@ -329,7 +340,9 @@ __device__ __forceinline__ void store_chunk(T* __restrict__ line, int e0, int n,
 // shuffles, and there is no LDS and no barrier.  Each segment (W lanes) owns
 // one line, a workgroup TS = WAVES*64/W lines.
 template <typename TIn, typename TOut, int R, int W, int WAVES, int PROD, int PREC, bool FULL>
-__global__ void __launch_bounds__(WAVES * 64) line_kernel_direct(const LineArgs args) {
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_direct(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
   constexpr int SPW = 64 / W;
   constexpr int TS = WAVES * SPW;
   constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
@@ -391,7 +404,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <typename TIn, typename TOut, int R, int W, int WAVES, int PROD, int PREC, bool FULL>
-__global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs args) {
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
   constexpr int SPW = 64 / W;                 // lines per wavefront
   constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
   constexpr int CS = R + 2;                   // LDS doubles per chunk (2 pad: bank spread)
@@ -533,7 +548,9 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
 // rows (small for W = 64 or 32).  One workgroup barrier separates loading and
 // copying each block; blocks alternate between two LDS buffers.
 template <typename TIn, typename TOut, int R, int W, int WAVES, int E, int PROD, int PREC, bool FULL>
-__global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs args) {
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
   constexpr int NT = WAVES * 64;
   constexpr int SPW = 64 / W;
   constexpr int TS = WAVES * SPW;             // lines per tile
@@ -583,34 +600,78 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs
   for (int k = 0; k < R; ++k) f[k] = 0.0;
   double bnd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 
-  for (int b = 0; b < nblk; ++b) {
+  // Block loads are software-pipelined: the values of block b+1 are fetched
+  // into registers (pf) before the barrier and copy of block b, so their
+  // latency overlaps the copy.  PF: values per thread per block.
+  constexpr int VPR = ROWS / (16 / static_cast<int>(sizeof(TIn)));   // vectors per block row
+  constexpr int PFV = (VPR * E + NT - 1) / NT;                        // vector loads per thread
+  constexpr int PFS = (ROWS * E + NT - 1) / NT;                       // scalar loads per thread
+  constexpr int VIN_ = 16 / static_cast<int>(sizeof(TIn));
+  constexpr int PF = (PFV * VIN_ > PFS) ? PFV * VIN_ : PFS;
+  double pf[PF];
+  auto fetch = [&](int b) {
+    const int e0 = b * E;
+    const int ecount = (n - e0 < E) ? n - e0 : E;
+    if (vec_in) {
+#pragma unroll
+      for (int i = 0; i < PFV; ++i) {
+        const int idx = tid + i * NT;
+        const int vr = idx % VPR, el = idx / VPR;
+        if (el < ecount) {
+          const TIn* src = in + static_cast<int64_t>(e0 + el) * args.estride + vr * VIN_;
+          if constexpr (sizeof(TIn) == 8) {
+            const nt_dbl2 t = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(src));
+            pf[i * 2] = t.x;
+            pf[i * 2 + 1] = t.y;
+          } else {
+            const nt_flt4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_flt4*>(src));
+            pf[i * 4] = t.x;
+            pf[i * 4 + 1] = t.y;
+            pf[i * 4 + 2] = t.z;
+            pf[i * 4 + 3] = t.w;
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * NT;
+        const int r = idx % ROWS, el = idx / ROWS;
+        if (el < ecount && r < rows_valid)
+          pf[i] = static_cast<double>(in[r + static_cast<int64_t>(e0 + el) * args.estride]);
+      }
+    }
+  };
+  auto deposit = [&](int b) {
     const int e0 = b * E;
     const int buf = b & 1;
     const int ecount = (n - e0 < E) ? n - e0 : E;
     if (vec_in) {
-      constexpr int VPR = ROWS / VIN;          // vectors per block row
-      for (int idx = tid; idx < VPR * ecount; idx += NT) {
+#pragma unroll
+      for (int i = 0; i < PFV; ++i) {
+        const int idx = tid + i * NT;
         const int vr = idx % VPR, el = idx / VPR;
-        const TIn* src = in + static_cast<int64_t>(e0 + el) * args.estride + vr * VIN;
-        if constexpr (sizeof(TIn) == 8) {
-          const nt_dbl2 t = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(src));
-          lds[at(buf, vr * 2, el)] = t.x;
-          lds[at(buf, vr * 2 + 1, el)] = t.y;
-        } else {
-          const nt_flt4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_flt4*>(src));
-          lds[at(buf, vr * 4, el)] = t.x;
-          lds[at(buf, vr * 4 + 1, el)] = t.y;
-          lds[at(buf, vr * 4 + 2, el)] = t.z;
-          lds[at(buf, vr * 4 + 3, el)] = t.w;
+        if (el < ecount) {
+#pragma unroll
+          for (int c = 0; c < VIN_; ++c) lds[at(buf, vr * VIN_ + c, el)] = pf[i * VIN_ + c];
         }
       }
     } else {
-      for (int idx = tid; idx < ROWS * ecount; idx += NT) {
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * NT;
         const int r = idx % ROWS, el = idx / ROWS;
-        if (r < rows_valid)
-          lds[at(buf, r, el)] = static_cast<double>(in[r + static_cast<int64_t>(e0 + el) * args.estride]);
+        if (el < ecount && r < rows_valid) lds[at(buf, r, el)] = pf[i];
       }
     }
+  };
+
+  fetch(0);
+  for (int b = 0; b < nblk; ++b) {
+    const int e0 = b * E;
+    const int buf = b & 1;
+    deposit(b);
+    if (b + 1 < nblk) fetch(b + 1);
     __syncthreads();
     if (my_blk == b && active) {
       const int el0 = my_e0 - e0;
@@ -636,8 +697,8 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs
         if (e >= e0 && e < e0 + E) bnd[i] = lds[at(buf, ql + HALO, e - e0)];
       }
     }
-    // the next block writes the other buffer; the one after that reuses this
-    // buffer only after the next iteration's barrier
+    // block b+1 is written to the other buffer; this buffer is rewritten (block
+    // b+2) only after the next barrier, which every reader of block b has passed
   }
 
   double d[R];

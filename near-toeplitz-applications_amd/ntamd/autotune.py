@@ -1,0 +1,107 @@
+"""Launch-shape autotuner for the line kernels.
+
+For each (layout, line length n) the library has several equivalent launch
+shapes (``nt_tune_set``): contiguous lines can stream through a wave-private
+LDS transpose, load per-lane chunks directly, or go through a workgroup tile;
+strided lines can use tiles of 16 or of 32 adjacent lines.  ``tune()`` times
+every variant on a synthetic field of the given shape class, keeps the fastest,
+and ``save()``/``apply()`` persist the choice in a JSON cache keyed by device
+name and library version.  Variants give the same results (the strided ones to
+within one rounding), so tuning never changes numerics beyond that.
+"""
+import ctypes
+import json
+import os
+
+import torch
+
+from . import _lib
+from .core import Plan, compact_derivative
+
+__all__ = ["tune", "apply", "save", "load", "current", "DEFAULT_CACHE"]
+
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+COMPACT = (1.0, 2.0, 0.25, 1.0, 0.25, 2.0, 1.0)
+
+
+def _device_key():
+    name = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
+    return f"{name}|abi{_lib.load().nt_version()}"
+
+
+def current(layout, n):
+    """(variant, number of variants) in use for lines of length n."""
+    v, nv = ctypes.c_int(), ctypes.c_int()
+    _lib.check(_lib.load().nt_tune_get(int(layout), ctypes.c_int64(int(n)), ctypes.byref(v), ctypes.byref(nv)))
+    return v.value, nv.value
+
+
+def _set(layout, n, variant):
+    _lib.check(_lib.load().nt_tune_set(int(layout), ctypes.c_int64(int(n)), int(variant)))
+
+
+def _time(fn, reps):
+    for _ in range(2):
+        fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps * 1e-3
+
+
+def tune(ns=(64, 128, 256, 512, 1024, 2048), points=1 << 25, reps=10, precision="fp64", verbose=False):
+    """Time every variant for contiguous (layout 0) and strided (layout 1) lines
+    of each length in ``ns`` on a field of about ``points`` points; apply and
+    return ``{"<layout>:<n>": {"variant": v, "seconds": [...]}}``."""
+    dev = torch.device("cuda")
+    result = {}
+    for n in ns:
+        other = max(1, int((points / n) ** 0.5))
+        for layout, axis, shape in ((0, 2, (other, other, n)), (1, 1, (other, n, other))):
+            f = torch.randn(shape, dtype=torch.float64, device=dev)
+            out = torch.empty_like(f)
+            plan = Plan(n, COMPACT)
+            _, nv = current(layout, n)
+            times = []
+            for v in range(nv):
+                _set(layout, n, v)
+                times.append(_time(lambda: compact_derivative(plan, f, out, 1.0 / (n - 1), axis, precision), reps))
+            best = min(range(nv), key=lambda i: times[i])
+            _set(layout, n, best)
+            result[f"{layout}:{n}"] = {"variant": best, "seconds": times}
+            if verbose:
+                print(f"layout {layout} n {n}: " + ", ".join(f"v{i} {t * 1e6:.1f}us" for i, t in enumerate(times)),
+                      flush=True)
+            del f, out
+    return result
+
+
+def save(result, path=DEFAULT_CACHE):
+    data = {}
+    if os.path.exists(path):
+        with open(path) as fh:
+            data = json.load(fh)
+    data[_device_key()] = {k: v["variant"] for k, v in result.items()}
+    with open(path, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+
+
+def load(path=DEFAULT_CACHE):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return json.load(fh).get(_device_key(), {})
+
+
+def apply(path=DEFAULT_CACHE):
+    """Set the cached variants for this device; returns what was applied."""
+    table = load(path)
+    for key, v in table.items():
+        layout, n = (int(s) for s in key.split(":"))
+        _set(layout, n, int(v))
+    return table

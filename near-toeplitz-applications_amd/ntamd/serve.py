@@ -1,0 +1,256 @@
+"""Continuous-batching server for compact-derivative and ADI requests.
+
+No reference counterpart: the reference runs one field at a time from its
+drivers (experiments/run-compact.py, run-adi.py).  A request is
+
+* ``deriv``: the compact derivative along ``axis`` of a seeded random-phase
+  trigonometric field of shape (nz, ny, nx), or
+* ``adi``: ``steps`` Peaceman-Rachford steps on a side x side grid at ratio r,
+
+each in FP64 or FP32 + refinement.  ``Server.run`` replays a request stream
+with its arrival times on the wall clock:
+
+* arrived requests wait until their field fits in the HBM budget (admission by
+  free HBM: the budget is a fraction of ``torch.cuda.mem_get_info`` minus what
+  admitted requests hold);
+* on every scheduling pass all newly admitted derivative requests go to the
+  GPU together as one mixed persistent launch (``MixedBatch``) on the
+  derivative stream, and each admitted ADI request is launched on the next of
+  the ADI streams, so new work overlaps work already on the device;
+* completions are observed with events; memory is released and the latency
+  (completion - arrival) recorded.
+
+Plans (the factored level tables) are cached by (n, coefficients) and fields
+stay on the device from admission to completion.  ``mode="sequential"`` is
+the baseline: one request at a time, plain FP64, each waited for before the
+next starts.  The numerics of a request do not depend on what it is batched
+with, so results are identical alone and in the mix.
+"""
+import collections
+import dataclasses
+import math
+import time
+
+import numpy as np
+import torch
+
+from .batch import MixedBatch
+from .core import Plan, adi_prepare, adi_steps, compact_derivative
+from . import adi_coeffs, COMPACT_COEFFS
+
+__all__ = ["Request", "make_requests", "Server", "trig_field", "adi_field", "percentile"]
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str                      # "deriv" | "adi"
+    arrival: float                 # seconds after the start of the stream
+    precision: str = "fp64"        # "fp64" | "fp32_refine"
+    seed: int = 0
+    shape: tuple = (0, 0, 0)       # deriv
+    axis: int = 2                  # deriv
+    side: int = 0                  # adi
+    steps: int = 0                 # adi
+    r: float = 0.0                 # adi
+    admitted: float = math.nan
+    done: float = math.nan
+    result: object = None
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+    def nbytes(self):
+        if self.kind == "deriv":
+            return int(np.prod(self.shape)) * 16
+        return self.side * self.side * 16
+
+
+def make_requests(count, rate, seed=2, deriv_share=0.5, fp32_share=0.5, deriv_sizes=(32, 512),
+                  adi_sides=(256, 2048), adi_steps=(1, 20), r_range=(0.1, 2.0)):
+    """Poisson arrivals at ``rate`` requests/s; per-request kind, sizes, axis,
+    precision and seeds drawn from numpy.random.default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    reqs = []
+    for i in range(count):
+        prec = "fp32_refine" if rng.random() < fp32_share else "fp64"
+        if rng.random() < deriv_share:
+            shape = tuple(int(s) for s in rng.integers(deriv_sizes[0], deriv_sizes[1] + 1, size=3))
+            reqs.append(Request(i, "deriv", float(t[i]), prec, int(rng.integers(1 << 30)), shape=shape,
+                                axis=int(rng.integers(0, 3))))
+        else:
+            reqs.append(Request(i, "adi", float(t[i]), prec, int(rng.integers(1 << 30)),
+                                side=int(rng.integers(adi_sides[0], adi_sides[1] + 1)),
+                                steps=int(rng.integers(adi_steps[0], adi_steps[1] + 1)),
+                                r=float(rng.uniform(*r_range))))
+    return reqs
+
+
+def trig_field(shape, seed, device):
+    """Seeded random-phase field sin(2pi(kx x + px)) sin(2pi(ky y + py)) sin(2pi(kz z + pz))
+    on [0,1]^3 with endpoints, wavenumbers 1..4."""
+    g = np.random.default_rng(seed)
+    k = g.integers(1, 5, size=3)
+    ph = g.random(3)
+    nz, ny, nx = shape
+    lin = lambda n: torch.linspace(0.0, 1.0, n, dtype=torch.float64, device=device)  # noqa: E731
+    sz = torch.sin(2 * math.pi * (k[0] * lin(nz) + ph[0]))
+    sy = torch.sin(2 * math.pi * (k[1] * lin(ny) + ph[1]))
+    sx = torch.sin(2 * math.pi * (k[2] * lin(nx) + ph[2]))
+    return sz[:, None, None] * sy[None, :, None] * sx[None, None, :]
+
+
+def adi_field(side, seed, device):
+    """Seeded initial condition for an ADI request: a Gaussian bump at a random
+    centre, zero Dirichlet boundary."""
+    g = np.random.default_rng(seed)
+    cx, cy = g.uniform(0.3, 0.7, size=2)
+    x = torch.linspace(0.0, 1.0, side, dtype=torch.float64, device=device)
+    u = torch.exp(-((x[None, :] - cx) ** 2 + (x[:, None] - cy) ** 2) / 0.01)
+    u[0, :] = 0.0
+    u[-1, :] = 0.0
+    u[:, 0] = 0.0
+    u[:, -1] = 0.0
+    return u.contiguous()
+
+
+def percentile(values, q):
+    return float(np.percentile(np.asarray(values, dtype=np.float64), q))
+
+
+class Server:
+
+    def __init__(self, device="cuda", hbm_fraction=0.5, hbm_budget=None, max_batch=64, adi_streams=2,
+                 keep_results=False):
+        self.device = torch.device(device)
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            hbm_budget = int(free * hbm_fraction)
+        self.budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self.keep_results = keep_results
+        self.deriv_stream = torch.cuda.Stream(device=self.device)
+        self.adi_stream_list = [torch.cuda.Stream(device=self.device) for _ in range(adi_streams)]
+        self._plans = {}
+        self._next_adi = 0
+
+    # -- plan cache (device-resident factored coefficients for the plain kernels) --
+    def plan(self, n, coeffs):
+        key = (int(n), tuple(coeffs))
+        p = self._plans.get(key)
+        if p is None:
+            p = self._plans[key] = Plan(n, coeffs)
+        return p
+
+    # -- launching ------------------------------------------------------------------
+    def _launch_derivs(self, reqs, batched, precision=None):
+        s = self.deriv_stream
+        with torch.cuda.stream(s):
+            jobs, keep = [], []
+            for r in reqs:
+                f = trig_field(r.shape, r.seed, self.device)
+                if f.dim() != 3 or not f.is_contiguous():
+                    f = f.contiguous()
+                out = torch.empty(r.shape, dtype=torch.float64, device=self.device)
+                h = 1.0 / (r.shape[r.axis] - 1)
+                jobs.append(dict(f=f, out=out, axis=r.axis, dx=h, precision=precision or r.precision,
+                                 sweeps=1))
+                keep.append((r, out))
+            if batched:
+                MixedBatch(jobs).run()
+            else:
+                for j in jobs:
+                    compact_derivative(self.plan(j["f"].shape[j["axis"]], COMPACT_COEFFS), j["f"], j["out"],
+                                       j["dx"], j["axis"], j["precision"], j["sweeps"])
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return ev, keep, jobs
+
+    def _launch_adi(self, r, precision=None):
+        s = self.adi_stream_list[self._next_adi % len(self.adi_stream_list)]
+        self._next_adi += 1
+        with torch.cuda.stream(s):
+            u = adi_field(r.side, r.seed, self.device)
+            work = torch.empty_like(u)
+            adi_prepare(u, work)
+            adi_steps(self.plan(r.side, adi_coeffs(r.r)), u, work, r.r, r.steps, precision or r.precision, 1)
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return ev, [(r, u)], [work]
+
+    # -- the scheduling loop ------------------------------------------------------------
+    def run(self, requests, mode="continuous", time_scale=1.0):
+        """Serve ``requests`` (arrival times in seconds, scaled by time_scale);
+        returns a stats dict.  mode: "continuous" | "sequential"."""
+        if mode not in ("continuous", "sequential"):
+            raise ValueError("mode must be 'continuous' or 'sequential'")
+        reqs = sorted(requests, key=lambda q: q.arrival)
+        for q in reqs:
+            q.done = q.admitted = math.nan
+            q.result = None
+        pending = collections.deque(reqs)
+        waiting = collections.deque()
+        inflight = []              # (event, [(request, result tensor)], scratch)
+        used = 0
+        batches = 0
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0  # noqa: E731
+        while pending or waiting or inflight:
+            now = clock()
+            while pending and pending[0].arrival * time_scale <= now:
+                waiting.append(pending.popleft())
+            progressed = False
+            still = []
+            for ev, items, scratch in inflight:
+                if ev.query():
+                    t = clock()
+                    for q, res in items:
+                        q.done = t
+                        used -= q.nbytes()
+                        if self.keep_results:
+                            q.result = res
+                    progressed = True
+                else:
+                    still.append((ev, items, scratch))
+            inflight = still
+            if mode == "continuous":
+                admitted = []
+                while waiting and len(admitted) < self.max_batch:
+                    q = waiting[0]
+                    need = q.nbytes()
+                    if used + need > self.budget and (used > 0 or admitted):
+                        break                       # wait for memory to free up
+                    waiting.popleft()
+                    used += need
+                    q.admitted = clock()
+                    admitted.append(q)
+                derivs = [q for q in admitted if q.kind == "deriv"]
+                for q in admitted:
+                    if q.kind == "adi":
+                        inflight.append(self._launch_adi(q))
+                if derivs:
+                    inflight.append(self._launch_derivs(derivs, batched=True))
+                    batches += 1
+                progressed = progressed or bool(admitted)
+            else:
+                if not inflight and waiting:
+                    q = waiting.popleft()
+                    used += q.nbytes()
+                    q.admitted = clock()
+                    if q.kind == "deriv":
+                        inflight.append(self._launch_derivs([q], batched=False, precision="fp64"))
+                    else:
+                        inflight.append(self._launch_adi(q, precision="fp64"))
+                    progressed = True
+            if not progressed:
+                time.sleep(20e-6)
+        torch.cuda.synchronize(self.device)
+        lat = [q.latency for q in reqs]
+        span = max(q.done for q in reqs) - min(q.arrival * time_scale for q in reqs)
+        return {"mode": mode, "requests": len(reqs), "p50_ms": 1e3 * percentile(lat, 50),
+                "p99_ms": 1e3 * percentile(lat, 99), "mean_ms": 1e3 * float(np.mean(lat)),
+                "throughput_rps": len(reqs) / span if span > 0 else math.inf, "batches": batches,
+                "budget_bytes": self.budget}

@@ -40,10 +40,12 @@ def compact_field_device(shape, seed=0, sigma=1e-3, dtype=torch.float64, device=
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     f = torch.empty((nz, ny, nx), dtype=dtype, device=dev)
-    for k in range(nz):   # one z-plane at a time keeps the FP64 temporaries small
-        plane = (sz[k] * sy[:, None] * sx[None, :])
-        plane = plane + sigma * torch.randn((ny, nx), generator=g, dtype=torch.float64, device=dev)
-        f[k].copy_(plane)
+    step = max(1, (1 << 26) // max(1, ny * nx))   # ~64 M points (512 MiB FP64) per chunk
+    for k0 in range(0, nz, step):
+        k1 = min(nz, k0 + step)
+        blk = sz[k0:k1, None, None] * sy[None, :, None] * sx[None, None, :]
+        blk += sigma * torch.randn((k1 - k0, ny, nx), generator=g, dtype=torch.float64, device=dev)
+        f[k0:k1].copy_(blk)
     return f
 
 
