@@ -228,19 +228,25 @@ def mixed_batch_case(count=64, seed=1, reps=3):
     return res
 
 
-def serving_case(count=200, rate=500.0, seed=2):
-    """Poisson arrivals (rate req/s): compact derivatives of random-phase trig
-    fields (per-axis sizes U[32,512], random axis) and ADI runs (side U[256,2048],
-    1-20 steps, r U[0.1,2]), FP64 or FP32+refinement each; continuous batching vs
-    one-at-a-time plain FP64."""
+def serving_case(count=1000, rates=(500.0, 2000.0), seed=2):
+    """Poisson arrivals (rate req/s, numpy default_rng(2)): compact derivatives
+    of random-phase trig fields (per-axis sizes U[32,512], random axis) and ADI
+    runs (side U[256,2048], 1-20 steps, r U[0.1,2]), FP64 or FP32+refinement
+    each; continuous batching vs one-at-a-time plain FP64, same request stream,
+    each on a warmed-up server."""
     from ntamd.serve import Server, make_requests
-    out = {"rate_rps": rate, "requests": count}
-    for mode in ("continuous", "sequential"):
-        reqs = make_requests(count, rate, seed=seed)
-        stats = Server().run(reqs, mode=mode)
-        out[mode] = stats
-        log(f"[bench] serving {mode}: p50 {stats['p50_ms']:.2f} ms, p99 {stats['p99_ms']:.2f} ms, "
-            f"{stats['throughput_rps']:.0f} req/s")
+    out = {"requests": count, "seed": seed}
+    for rate in rates:
+        case = {}
+        for mode in ("continuous", "sequential"):
+            reqs = make_requests(count, rate, seed=seed)
+            srv = Server()
+            srv.warmup()                 # first-launch costs are paid outside the timed stream
+            stats = srv.run(reqs, mode=mode)
+            case[mode] = stats
+            log(f"[bench] serving {rate:.0f} req/s {mode}: p50 {stats['p50_ms']:.2f} ms, "
+                f"p99 {stats['p99_ms']:.2f} ms, {stats['throughput_rps']:.0f} req/s")
+        out[f"rate_{rate:.0f}"] = case
     return out
 
 

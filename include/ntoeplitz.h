@@ -45,11 +45,13 @@ int nt_version(void);
 /* ---- launch tuning (no reference counterpart: the reference hard-codes one
  * block per system, neato.py:181-183) --------------------------------------
  * Launch-shape variant used for lines of length n (bucketed by ceil(log2 n))
- * in layout 0 (contiguous lines) or 1 (strided lines).  Process-wide; set by
- * the autotuner (ntamd/autotune.py) from its JSON cache.  Every variant gives
- * the same results; only the launch shape changes. */
-int nt_tune_set(int layout, int64_t n, int variant);
-int nt_tune_get(int layout, int64_t n, int* variant, int* nvariants);
+ * in layout 0 (contiguous lines) or 1 (strided lines) at the given precision
+ * mode (NT_PREC_*).  Process-wide; the defaults are the shapes measured
+ * fastest on MI355X, and the autotuner (ntamd/autotune.py) can re-measure and
+ * set them from its JSON cache.  Every variant gives the same results to
+ * within one rounding; only the launch shape changes. */
+int nt_tune_set(int layout, int precision, int64_t n, int variant);
+int nt_tune_get(int layout, int precision, int64_t n, int* variant, int* nvariants);
 
 /* ---- plans ---------------------------------------------------------------
  * Replaces NearToeplitzSolver.__init__(n, nrhs, coeffs) (neato.py:131-166):

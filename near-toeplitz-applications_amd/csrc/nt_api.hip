@@ -34,8 +34,8 @@ int nt::set_error(int code, const char* fmt, ...) {
 }
 
 namespace {
-// launch-variant table: [layout][ceil(log2 n)]
-int g_tune[2][13] = {};
+// launch-variant table: [layout][precision][ceil(log2 n)]
+int g_tune[2][2][13] = {};
 
 int log2_bucket(int n) {
   int b = 0;
@@ -44,8 +44,10 @@ int log2_bucket(int n) {
 }
 }  // namespace
 
-int nt::tune_get(int layout, int n) { return g_tune[layout & 1][log2_bucket(n)]; }
-void nt::tune_set(int layout, int n, int variant) { g_tune[layout & 1][log2_bucket(n)] = variant; }
+int nt::tune_get(int layout, int prec, int n) { return g_tune[layout & 1][prec & 1][log2_bucket(n)]; }
+void nt::tune_set(int layout, int prec, int n, int variant) {
+  g_tune[layout & 1][prec & 1][log2_bucket(n)] = variant;
+}
 
 namespace {
 
@@ -120,20 +122,25 @@ const char* nt_last_error(void) { return g_err; }
 
 int nt_version(void) { return 100; }
 
-int nt_tune_set(int layout, int64_t n, int variant) {
+int nt_tune_set(int layout, int precision, int64_t n, int variant) {
   if (layout != 0 && layout != 1) return fail(NT_EINVAL, "nt_tune_set: layout must be 0 or 1");
+  if (precision != NT_PREC_F64 && precision != NT_PREC_F32_REFINE)
+    return fail(NT_EINVAL, "nt_tune_set: precision %d", precision);
   if (n < 2 || n > NT_MAX_N) return fail(NT_EINVAL, "nt_tune_set: n out of range");
   if (variant < 0 || variant >= nt::kNumVariants[layout])
     return fail(NT_EINVAL, "nt_tune_set: variant %d out of range [0, %d)", variant,
                 nt::kNumVariants[layout]);
-  nt::tune_set(layout, static_cast<int>(n), variant);
+  nt::tune_set(layout, precision == NT_PREC_F64 ? nt::PREC_F64 : nt::PREC_F32R, static_cast<int>(n), variant);
   return ok();
 }
 
-int nt_tune_get(int layout, int64_t n, int* variant, int* nvariants) {
+int nt_tune_get(int layout, int precision, int64_t n, int* variant, int* nvariants) {
   if (layout != 0 && layout != 1) return fail(NT_EINVAL, "nt_tune_get: layout must be 0 or 1");
+  if (precision != NT_PREC_F64 && precision != NT_PREC_F32_REFINE)
+    return fail(NT_EINVAL, "nt_tune_get: precision %d", precision);
   if (n < 2 || n > NT_MAX_N) return fail(NT_EINVAL, "nt_tune_get: n out of range");
-  if (variant) *variant = nt::tune_get(layout, static_cast<int>(n));
+  if (variant)
+    *variant = nt::tune_get(layout, precision == NT_PREC_F64 ? nt::PREC_F64 : nt::PREC_F32R, static_cast<int>(n));
   if (nvariants) *nvariants = nt::kNumVariants[layout];
   return ok();
 }

@@ -6,8 +6,10 @@
 // device and zeroes a work counter; one persistent launch then pulls tiles of
 // 16 lines from the counter until every job is done.  A tile is 16 lines
 // adjacent in memory (16 rows of an x-line field, or 16 adjacent y/z lines),
-// one wavefront per line, staged through LDS; each wavefront runs the same
-// register CR as the other kernels (general-n path, up to 512 points).
+// staged through LDS with all of a thread's loads in flight at once; each of
+// the 8 wavefronts then solves two of the lines with the same register CR as
+// the other kernels, 64 lanes x R rows with R = 1, 2, 4 or 8 by line length
+// (up to 512 points).
 // Per-line arithmetic does not depend on which other jobs share the launch, so
 // a field gives the same bits alone and in any mix.
 #include <hip/hip_runtime.h>
@@ -23,9 +25,13 @@
 namespace nt {
 
 constexpr int BATCH_MAXN = 512;
-constexpr int BATCH_R = 8;                 // 64 lanes x 8 rows
-constexpr int BATCH_TS = 8;                // lines per tile = wavefronts per workgroup
-constexpr int BATCH_EP = BATCH_MAXN + 2;   // LDS stride between lines (doubles)
+constexpr int BATCH_TS = 16;                // lines per tile
+constexpr int BATCH_WAVES = 8;              // wavefronts per workgroup
+constexpr int BATCH_LPW = BATCH_TS / BATCH_WAVES;   // lines solved by each wavefront
+constexpr int BATCH_NT = BATCH_WAVES * 64;
+constexpr int BATCH_EP = BATCH_MAXN + 2;    // LDS stride between lines (doubles)
+// every thread moves BATCH_TS * BATCH_MAXN / BATCH_NT elements of a tile
+constexpr int BATCH_PER = BATCH_TS * BATCH_MAXN / BATCH_NT;
 
 struct BatchJob {
   const void* in;
@@ -53,15 +59,82 @@ struct BatchArgs {
   int32_t pad_;
 };
 
-__global__ void __launch_bounds__(BATCH_TS * 64) batch_kernel(const BatchArgs args) {
+// Element k of this thread's share of a tile: (line, position) in the tile.
+// Contiguous lines: wavefront w moves lines w, w + WAVES, ...; lane l takes
+// positions l, l + 64, ... (each wavefront reads whole 512 B row runs).
+// Strided lines: the TS lines are adjacent in memory, so TS consecutive
+// threads read one TS * 8 B run of a row; thread t takes line t % TS,
+// positions t / TS + k * (NT / TS).
+__device__ __forceinline__ void tile_slot(int layout, int tid, int k, int& line, int& e) {
+  if (layout == CONTIG) {
+    constexpr int PER_LINE = BATCH_MAXN / 64;        // positions per lane per line
+    line = (tid >> 6) + (k / PER_LINE) * BATCH_WAVES;
+    e = (tid & 63) + (k % PER_LINE) * 64;
+  } else {
+    line = tid % BATCH_TS;
+    e = tid / BATCH_TS + k * (BATCH_NT / BATCH_TS);
+  }
+}
+
+template <typename TIn>
+__device__ __forceinline__ void stage_tile(double* lds, const TIn* __restrict__ in, int layout, int tid, int rows,
+                                           int n, int64_t base, int64_t pstride, int64_t estride) {
+  double v[BATCH_PER];
+  // issue every load of the tile before the first LDS write so they overlap
+#pragma unroll
+  for (int k = 0; k < BATCH_PER; ++k) {
+    int line, e;
+    tile_slot(layout, tid, k, line, e);
+    v[k] = (line < rows && e < n)
+               ? static_cast<double>(__builtin_nontemporal_load(in + base + line * pstride + e * estride))
+               : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < BATCH_PER; ++k) {
+    int line, e;
+    tile_slot(layout, tid, k, line, e);
+    if (line < rows && e < n) lds[line * BATCH_EP + e] = v[k];
+  }
+}
+
+// One line held in LDS row `row` (n points): lane l takes points l R .. l R + R - 1,
+// builds the right-hand side (compact producer) or takes the values as they are
+// (solve), runs the register CR and writes the solution back over the row.
+template <int R, bool FULL>
+__device__ __forceinline__ void solve_line(double* row, int n, int prod, int prec, const NtPlanK& P, double s1,
+                                           double s2, int sweeps, int lane) {
+  const int my_e0 = lane * R;
+  double f[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k] = (FULL || my_e0 + k < n) ? row[my_e0 + k] : 0.0;
+  double d[R];
+  if (prod == P_COMPACT) {
+    double bnd[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) bnd[i] = row[(i < 3) ? i : n + 2 - i];
+    const double fl = shup<64>(f[R - 1], 1);
+    const double fr = shdn<64>(f[0], 1);
+    compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, s1, s2, d);
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) d[k] = f[k];
+  }
+  double x[R];
+  if (prec == PREC_F64) solve_chunk<R, 64, PREC_F64, FULL>(d, x, P, lane, sweeps);
+  else solve_chunk<R, 64, PREC_F32R, FULL>(d, x, P, lane, sweeps);
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+    if (FULL || my_e0 + k < n) row[my_e0 + k] = x[k];
+}
+
+__global__ void __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
+batch_kernel(const BatchArgs args) {
   __shared__ __attribute__((aligned(16))) double lds[BATCH_TS * BATCH_EP];
   __shared__ __attribute__((aligned(16))) NtPlanK s_plan;   // level table of the current job
   __shared__ int64_t s_tile;
   const int tid = threadIdx.x;
-  const int ql = tid >> 6;          // my line within the tile
+  const int wave = tid >> 6;
   const int lane = tid & 63;
-  constexpr int R = BATCH_R;
-  constexpr int NT = BATCH_TS * 64;
   int j = 0;                        // current job: tiles come in increasing order
   int plan_in_lds = -1;
 
@@ -78,7 +151,7 @@ __global__ void __launch_bounds__(BATCH_TS * 64) batch_kernel(const BatchArgs ar
     if (plan != plan_in_lds) {                   // copy the level table into LDS
       const int* src = reinterpret_cast<const int*>(args.plans + plan);
       int* dst = reinterpret_cast<int*>(&s_plan);
-      for (int i = tid; i < static_cast<int>(sizeof(NtPlanK) / 4); i += NT) dst[i] = src[i];
+      for (int i = tid; i < static_cast<int>(sizeof(NtPlanK) / 4); i += BATCH_NT) dst[i] = src[i];
       plan_in_lds = plan;
     }
     const int64_t local = t - J.tile0;
@@ -87,73 +160,55 @@ __global__ void __launch_bounds__(BATCH_TS * 64) batch_kernel(const BatchArgs ar
     const int64_t base = o * J.ostride + p0 * J.pstride;
     const int64_t rows64 = J.phys - p0;
     const int rows = rows64 < BATCH_TS ? static_cast<int>(rows64) : BATCH_TS;
-    const int layout = uni(J.layout), in_f32 = uni(J.in_f32), prod = uni(J.prod), prec = uni(J.prec);
+    const int layout = uni(J.layout), prod = uni(J.prod), prec = uni(J.prec);
     const int n = uni(args.plans[plan].n);
     const int64_t pstride = J.pstride, estride = J.estride;
-    const void* in = J.in;
     double* __restrict__ out = static_cast<double*>(J.out);
 
-    // ---- stage the tile (lines x n) through LDS, coalesced along memory ----------
-    if (layout == CONTIG) {
-      if (ql < rows) {                           // wavefront ql loads its own line
-        const int64_t g0 = base + ql * pstride;
-        for (int e = lane; e < n; e += 64)
-          lds[ql * BATCH_EP + e] = in_f32 ? static_cast<double>(static_cast<const float*>(in)[g0 + e])
-                                          : static_cast<const double*>(in)[g0 + e];
-      }
-    } else {
-      for (int idx = tid; idx < BATCH_TS * n; idx += NT) {
-        const int r = idx % BATCH_TS, e = idx / BATCH_TS;
-        if (r < rows) {
-          const int64_t g = base + r + static_cast<int64_t>(e) * estride;
-          lds[r * BATCH_EP + e] = in_f32 ? static_cast<double>(static_cast<const float*>(in)[g])
-                                         : static_cast<const double*>(in)[g];
+    // ---- stage the tile (TS lines x n) through LDS, coalesced along memory ------
+    if (uni(J.in_f32))
+      stage_tile(lds, static_cast<const float*>(J.in), layout, tid, rows, n, base, pstride, estride);
+    else
+      stage_tile(lds, static_cast<const double*>(J.in), layout, tid, rows, n, base, pstride, estride);
+    __syncthreads();
+
+    // ---- each wavefront solves its lines; a line's LDS row is read and then
+    // overwritten by the same wavefront only, so no barrier in between.  The
+    // rows per lane follow the line length (64 lanes x R >= n), with the
+    // compile-time level structure when n == 64 R. -------------------------------
+    const NtPlanK& P = s_plan;
+    const int rsel = (n <= 64) ? 0 : (n <= 128) ? 1 : (n <= 256) ? 2 : 3;
+    const bool full = (n == (64 << rsel));
+#pragma unroll 1
+    for (int h = 0; h < BATCH_LPW; ++h) {
+      const int ql = wave + h * BATCH_WAVES;
+      if (ql >= rows) break;                     // uniform across the wavefront
+      double* row = lds + ql * BATCH_EP;
+      if (full) {
+        switch (rsel) {
+          case 0: solve_line<1, true>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+          case 1: solve_line<2, true>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+          case 2: solve_line<4, true>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+          default: solve_line<8, true>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+        }
+      } else {
+        switch (rsel) {
+          case 0: solve_line<1, false>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+          case 1: solve_line<2, false>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+          case 2: solve_line<4, false>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+          default: solve_line<8, false>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
         }
       }
     }
     __syncthreads();
 
-    // ---- my line: chunk, RHS, solve ---------------------------------------------------
-    const NtPlanK& P = s_plan;
-    const bool active = ql < rows;
-    const int my_e0 = lane * R;
-    double f[R];
+    // ---- store ----------------------------------------------------------------------
 #pragma unroll
-    for (int k = 0; k < R; ++k) f[k] = (active && my_e0 + k < n) ? lds[ql * BATCH_EP + my_e0 + k] : 0.0;
-    double d[R];
-    if (prod == P_COMPACT) {
-      double bnd[6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) bnd[i] = lds[ql * BATCH_EP + ((i < 3) ? i : n + 2 - i)];
-      const double fl = shup<64>(f[R - 1], 1);
-      const double fr = shdn<64>(f[0], 1);
-      compact_rhs_chunk<R, false>(f, fl, fr, bnd, my_e0, n, J.s1, J.s2, d);
-    } else {
-#pragma unroll
-      for (int k = 0; k < R; ++k) d[k] = f[k];
-    }
-    double x[R];
-    if (prec == PREC_F64) solve_chunk<R, 64, PREC_F64, false>(d, x, P, lane, J.sweeps);
-    else solve_chunk<R, 64, PREC_F32R, false>(d, x, P, lane, J.sweeps);
-    __syncthreads();   // every line finished reading the tile
-    if (active) {
-#pragma unroll
-      for (int k = 0; k < R; ++k)
-        if (my_e0 + k < n) lds[ql * BATCH_EP + my_e0 + k] = x[k];
-    }
-    __syncthreads();
-
-    // ---- store --------------------------------------------------------------------
-    if (layout == CONTIG) {
-      if (ql < rows) {
-        const int64_t g0 = base + ql * pstride;
-        for (int e = lane; e < n; e += 64) out[g0 + e] = lds[ql * BATCH_EP + e];
-      }
-    } else {
-      for (int idx = tid; idx < BATCH_TS * n; idx += NT) {
-        const int r = idx % BATCH_TS, e = idx / BATCH_TS;
-        if (r < rows) out[base + r + static_cast<int64_t>(e) * estride] = lds[r * BATCH_EP + e];
-      }
+    for (int k = 0; k < BATCH_PER; ++k) {
+      int line, e;
+      tile_slot(layout, tid, k, line, e);
+      if (line < rows && e < n)
+        __builtin_nontemporal_store(lds[line * BATCH_EP + e], out + base + line * pstride + e * estride);
     }
     __syncthreads();   // LDS, the plan copy and s_tile are reused by the next tile
   }
@@ -273,7 +328,7 @@ int nt_batch_launch(const void* dev_image, int njobs, int nplans, int64_t total_
   a.total_tiles = total_tiles;
   a.njobs = njobs;
   a.pad_ = 0;
-  hipLaunchKernelGGL(nt::batch_kernel, dim3(g), dim3(nt::BATCH_TS * 64), 0,
+  hipLaunchKernelGGL(nt::batch_kernel, dim3(g), dim3(nt::BATCH_NT), 0,
                      static_cast<hipStream_t>(stream), a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return nt::set_error(NT_EHIP, "nt_batch_launch: %s", hipGetErrorString(e));

@@ -14,7 +14,7 @@ namespace nt {
 template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
 hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 
-// Launch-shape variants, chosen per (layout, ceil(log2 n)) by the tuning table
+// Launch-shape variants, chosen per (layout, precision, ceil(log2 n)) by the tuning table
 // (nt_tune_set; default 0).  Every variant computes the same values.
 //   CONTIG  0: one line per wavefront, wave-coalesced nontemporal loads and
 //              stores, wave-private LDS transpose (needs 16-byte aligned lines)
@@ -25,8 +25,8 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads)
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 constexpr int kNumVariants[2] = {3, 2};
-int tune_get(int layout, int n);
-void tune_set(int layout, int n, int variant);
+int tune_get(int layout, int prec, int n);
+void tune_set(int layout, int prec, int n, int variant);
 
 // n == W*R with n a power of two: the level structure is known at compile time
 inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) == 0; }
@@ -96,7 +96,7 @@ inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
 template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
 hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
   const int n = a.plan.n;
-  const int variant = tune_get(LAYOUT, n);
+  const int variant = tune_get(LAYOUT, PREC, n);
   if constexpr (LAYOUT == CONTIG) {
     const int vin = 16 / static_cast<int>(sizeof(TIn)), vout = 16 / static_cast<int>(sizeof(TOut));
     const bool aligned = n > 64 && direct_ok(a, sizeof(TIn), sizeof(TOut));

@@ -34,8 +34,8 @@ _intp = ctypes.POINTER(ctypes.c_int)
 SIGNATURES = {
     "nt_last_error": (ctypes.c_char_p, []),
     "nt_version": (_int, []),
-    "nt_tune_set": (_int, [_int, _i64, _int]),
-    "nt_tune_get": (_int, [_int, _i64, _intp, _intp]),
+    "nt_tune_set": (_int, [_int, _int, _i64, _int]),
+    "nt_tune_get": (_int, [_int, _int, _i64, _intp, _intp]),
     "nt_plan_create": (_int, [_i64, _dp, _pp]),
     "nt_plan_destroy": (_int, [_vp]),
     "nt_plan_info": (_int, [_vp, _i64p, _intp]),

@@ -1,6 +1,6 @@
 """Launch-shape autotuner for the line kernels.
 
-For each (layout, line length n) the library has several equivalent launch
+For each (layout, precision mode, line length n) the library has several equivalent launch
 shapes (``nt_tune_set``): contiguous lines can stream through a wave-private
 LDS transpose, load per-lane chunks directly, or go through a workgroup tile;
 strided lines can use tiles of 16 or of 32 adjacent lines.  ``tune()`` times
@@ -16,7 +16,7 @@ import os
 import torch
 
 from . import _lib
-from .core import Plan, compact_derivative
+from .core import Plan, compact_derivative, precision_code
 
 __all__ = ["tune", "apply", "save", "load", "current", "DEFAULT_CACHE"]
 
@@ -29,15 +29,16 @@ def _device_key():
     return f"{name}|abi{_lib.load().nt_version()}"
 
 
-def current(layout, n):
+def current(layout, n, precision="fp64"):
     """(variant, number of variants) in use for lines of length n."""
     v, nv = ctypes.c_int(), ctypes.c_int()
-    _lib.check(_lib.load().nt_tune_get(int(layout), ctypes.c_int64(int(n)), ctypes.byref(v), ctypes.byref(nv)))
+    _lib.check(_lib.load().nt_tune_get(int(layout), precision_code(precision), ctypes.c_int64(int(n)),
+                                       ctypes.byref(v), ctypes.byref(nv)))
     return v.value, nv.value
 
 
-def _set(layout, n, variant):
-    _lib.check(_lib.load().nt_tune_set(int(layout), ctypes.c_int64(int(n)), int(variant)))
+def _set(layout, precision, n, variant):
+    _lib.check(_lib.load().nt_tune_set(int(layout), precision_code(precision), ctypes.c_int64(int(n)), int(variant)))
 
 
 def _time(fn, reps):
@@ -54,29 +55,32 @@ def _time(fn, reps):
     return a.elapsed_time(b) / reps * 1e-3
 
 
-def tune(ns=(64, 128, 256, 512, 1024, 2048), points=1 << 25, reps=10, precision="fp64", verbose=False):
+def tune(ns=(64, 128, 256, 512, 1024, 2048), points=1 << 25, reps=10, precisions=("fp64", "fp32_refine"),
+         verbose=False):
     """Time every variant for contiguous (layout 0) and strided (layout 1) lines
-    of each length in ``ns`` on a field of about ``points`` points; apply and
-    return ``{"<layout>:<n>": {"variant": v, "seconds": [...]}}``."""
+    of each length in ``ns``, at each precision mode, on a field of about
+    ``points`` points; apply and return
+    ``{"<layout>:<precision>:<n>": {"variant": v, "seconds": [...]}}``."""
     dev = torch.device("cuda")
     result = {}
     for n in ns:
         other = max(1, int((points / n) ** 0.5))
+        plan = Plan(n, COMPACT)
         for layout, axis, shape in ((0, 2, (other, other, n)), (1, 1, (other, n, other))):
             f = torch.randn(shape, dtype=torch.float64, device=dev)
             out = torch.empty_like(f)
-            plan = Plan(n, COMPACT)
-            _, nv = current(layout, n)
-            times = []
-            for v in range(nv):
-                _set(layout, n, v)
-                times.append(_time(lambda: compact_derivative(plan, f, out, 1.0 / (n - 1), axis, precision), reps))
-            best = min(range(nv), key=lambda i: times[i])
-            _set(layout, n, best)
-            result[f"{layout}:{n}"] = {"variant": best, "seconds": times}
-            if verbose:
-                print(f"layout {layout} n {n}: " + ", ".join(f"v{i} {t * 1e6:.1f}us" for i, t in enumerate(times)),
-                      flush=True)
+            for prec in precisions:
+                _, nv = current(layout, n, prec)
+                times = []
+                for v in range(nv):
+                    _set(layout, prec, n, v)
+                    times.append(_time(lambda: compact_derivative(plan, f, out, 1.0 / (n - 1), axis, prec), reps))
+                best = min(range(nv), key=lambda i: times[i])
+                _set(layout, prec, n, best)
+                result[f"{layout}:{prec}:{n}"] = {"variant": best, "seconds": times}
+                if verbose:
+                    print(f"layout {layout} {prec} n {n}: " +
+                          ", ".join(f"v{i} {t * 1e6:.1f}us" for i, t in enumerate(times)), flush=True)
             del f, out
     return result
 
@@ -102,6 +106,6 @@ def apply(path=DEFAULT_CACHE):
     """Set the cached variants for this device; returns what was applied."""
     table = load(path)
     for key, v in table.items():
-        layout, n = (int(s) for s in key.split(":"))
-        _set(layout, n, int(v))
+        layout, prec, n = key.split(":")
+        _set(int(layout), prec, int(n), int(v))
     return table

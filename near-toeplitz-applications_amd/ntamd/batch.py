@@ -86,13 +86,28 @@ class MixedBatch:
                                      ctypes.c_size_t(nbytes), ctypes.byref(total)))
         self.total_tiles = total.value
         self.njobs, self.nplans = len(jobs), len(plans)
-        self.image = torch.from_numpy(host).to(dev)              # device copy of the image
+        # asynchronous upload from pinned memory on the current stream; the
+        # pinned copy lives as long as the batch, and run() orders itself
+        # after the upload whatever stream it is called on
+        self._pinned = torch.from_numpy(host).pin_memory()
+        self.image = self._pinned.to(dev, non_blocking=True)
         if self.image.data_ptr() % 256:
             raise RuntimeError("device image is not 256-byte aligned")
         self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self._uploaded = torch.cuda.Event()
+        self._uploaded.record()
         self.grid = int(grid)
 
     def run(self):
+        ev = self._uploaded
+        if ev is not None:
+            if ev.query():
+                self._uploaded = None
+            elif torch.cuda.is_current_stream_capturing():
+                ev.synchronize()                  # a capture may not wait on an outside event
+                self._uploaded = None
+            else:
+                torch.cuda.current_stream().wait_event(ev)
         self.counter.zero_()
         _lib.call("nt_batch_launch", ctypes.c_void_p(self.image.data_ptr()), self.njobs, self.nplans,
                   ctypes.c_int64(self.total_tiles), ctypes.c_void_p(self.counter.data_ptr()),

@@ -13,10 +13,11 @@ with its arrival times on the wall clock:
 * arrived requests wait until their field fits in the HBM budget (admission by
   free HBM: the budget is a fraction of ``torch.cuda.mem_get_info`` minus what
   admitted requests hold);
-* on every scheduling pass all newly admitted derivative requests go to the
-  GPU together as one mixed persistent launch (``MixedBatch``) on the
-  derivative stream, and each admitted ADI request is launched on the next of
-  the ADI streams, so new work overlaps work already on the device;
+* on every scheduling pass the newly admitted small derivative requests go to
+  the GPU together as one mixed persistent launch (``MixedBatch``); larger
+  derivatives and ADI runs get their own fused launches; every launch group
+  goes to the least busy of a pool of streams, so new work overlaps work
+  already on the device;
 * completions are observed with events; memory is released and the latency
   (completion - arrival) recorded.
 
@@ -121,22 +122,31 @@ def percentile(values, q):
 
 
 class Server:
+    """Replays a request stream on one device.
 
-    def __init__(self, device="cuda", hbm_fraction=0.5, hbm_budget=None, max_batch=64, adi_streams=2,
-                 keep_results=False):
+    ``streams`` HIP streams (default priority) carry the work; each launch
+    group goes to the stream with the fewest groups in flight.  Derivative
+    requests of at most ``small_points`` points admitted in the same pass are
+    packed into one ``MixedBatch`` launch (there the per-launch cost dominates;
+    a lone small request is a batch of one); larger ones, and ADI runs, get
+    their own fused launches.
+    """
+
+    def __init__(self, device="cuda", hbm_fraction=0.5, hbm_budget=None, max_batch=64, streams=4,
+                 small_points=1 << 21, keep_results=False):
         self.device = torch.device(device)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             hbm_budget = int(free * hbm_fraction)
         self.budget = int(hbm_budget)
         self.max_batch = int(max_batch)
+        self.small_points = int(small_points)
         self.keep_results = keep_results
-        self.deriv_stream = torch.cuda.Stream(device=self.device)
-        self.adi_stream_list = [torch.cuda.Stream(device=self.device) for _ in range(adi_streams)]
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, int(streams)))]
+        self._busy = [0] * len(self.streams)
         self._plans = {}
-        self._next_adi = 0
 
-    # -- plan cache (device-resident factored coefficients for the plain kernels) --
+    # -- plan cache (host level tables; shared by all streams) ----------------------
     def plan(self, n, coeffs):
         key = (int(n), tuple(coeffs))
         p = self._plans.get(key)
@@ -144,41 +154,65 @@ class Server:
             p = self._plans[key] = Plan(n, coeffs)
         return p
 
-    # -- launching ------------------------------------------------------------------
+    def _batchable(self, q):
+        """Small derivative whose lines fit the persistent batch kernel (16..512)."""
+        return (q.kind == "deriv" and math.prod(q.shape) <= self.small_points
+                and 16 <= q.shape[q.axis] <= 512)
+
+    def _stream(self):
+        i = min(range(len(self.streams)), key=lambda k: self._busy[k])
+        self._busy[i] += 1
+        return i
+
+    # -- launching: each returns (event, stream index, [(request, result)], keep-alive) --
     def _launch_derivs(self, reqs, batched, precision=None):
-        s = self.deriv_stream
-        with torch.cuda.stream(s):
-            jobs, keep = [], []
+        si = self._stream()
+        with torch.cuda.stream(self.streams[si]):
+            jobs, items = [], []
             for r in reqs:
-                f = trig_field(r.shape, r.seed, self.device)
-                if f.dim() != 3 or not f.is_contiguous():
-                    f = f.contiguous()
+                f = trig_field(r.shape, r.seed, self.device).contiguous()
                 out = torch.empty(r.shape, dtype=torch.float64, device=self.device)
-                h = 1.0 / (r.shape[r.axis] - 1)
-                jobs.append(dict(f=f, out=out, axis=r.axis, dx=h, precision=precision or r.precision,
-                                 sweeps=1))
-                keep.append((r, out))
+                jobs.append(dict(f=f, out=out, axis=r.axis, dx=1.0 / (r.shape[r.axis] - 1),
+                                 precision=precision or r.precision, sweeps=1))
+                items.append((r, out))
+            keep = jobs
             if batched:
-                MixedBatch(jobs).run()
+                batch = MixedBatch(jobs)
+                batch.run()
+                keep = (jobs, batch)
             else:
                 for j in jobs:
                     compact_derivative(self.plan(j["f"].shape[j["axis"]], COMPACT_COEFFS), j["f"], j["out"],
                                        j["dx"], j["axis"], j["precision"], j["sweeps"])
             ev = torch.cuda.Event()
-            ev.record(s)
-        return ev, keep, jobs
+            ev.record()
+        return ev, si, items, keep
 
     def _launch_adi(self, r, precision=None):
-        s = self.adi_stream_list[self._next_adi % len(self.adi_stream_list)]
-        self._next_adi += 1
-        with torch.cuda.stream(s):
+        si = self._stream()
+        with torch.cuda.stream(self.streams[si]):
             u = adi_field(r.side, r.seed, self.device)
             work = torch.empty_like(u)
             adi_prepare(u, work)
             adi_steps(self.plan(r.side, adi_coeffs(r.r)), u, work, r.r, r.steps, precision or r.precision, 1)
             ev = torch.cuda.Event()
-            ev.record(s)
-        return ev, [(r, u)], [work]
+            ev.record()
+        return ev, si, [(r, u)], work
+
+    def warmup(self):
+        """Send one small request down every route (batched and fused derivative on
+        each axis, ADI; both precisions) through both modes, so that one-time costs
+        (code-object loading on first launch, allocator growth, pinned-memory pool)
+        are paid before a timed run."""
+        side = min(512, int(round(self.small_points ** (1.0 / 3.0))) + 2)
+        reqs = []
+        for prec in ("fp64", "fp32_refine"):
+            for axis in range(3):
+                reqs.append(Request(len(reqs), "deriv", 0.0, prec, 1, shape=(32, 32, 32), axis=axis))
+                reqs.append(Request(len(reqs), "deriv", 0.0, prec, 2, shape=(side, side, side), axis=axis))
+            reqs.append(Request(len(reqs), "adi", 0.0, prec, 3, side=256, steps=2, r=0.5))
+        for mode in ("continuous", "sequential"):
+            self.run(reqs, mode=mode)
 
     # -- the scheduling loop ------------------------------------------------------------
     def run(self, requests, mode="continuous", time_scale=1.0):
@@ -192,9 +226,10 @@ class Server:
             q.result = None
         pending = collections.deque(reqs)
         waiting = collections.deque()
-        inflight = []              # (event, [(request, result tensor)], scratch)
+        inflight = []              # (event, stream index, [(request, result tensor)], keep-alive)
         used = 0
         batches = 0
+        self._busy = [0] * len(self.streams)
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
@@ -204,7 +239,7 @@ class Server:
                 waiting.append(pending.popleft())
             progressed = False
             still = []
-            for ev, items, scratch in inflight:
+            for ev, si, items, keep in inflight:
                 if ev.query():
                     t = clock()
                     for q, res in items:
@@ -212,9 +247,10 @@ class Server:
                         used -= q.nbytes()
                         if self.keep_results:
                             q.result = res
+                    self._busy[si] -= 1
                     progressed = True
                 else:
-                    still.append((ev, items, scratch))
+                    still.append((ev, si, items, keep))
             inflight = still
             if mode == "continuous":
                 admitted = []
@@ -227,12 +263,14 @@ class Server:
                     used += need
                     q.admitted = clock()
                     admitted.append(q)
-                derivs = [q for q in admitted if q.kind == "deriv"]
+                small = [q for q in admitted if self._batchable(q)]
                 for q in admitted:
                     if q.kind == "adi":
                         inflight.append(self._launch_adi(q))
-                if derivs:
-                    inflight.append(self._launch_derivs(derivs, batched=True))
+                    elif not self._batchable(q):
+                        inflight.append(self._launch_derivs([q], batched=False))
+                if small:   # the route depends on the request only, so alone == in the mix
+                    inflight.append(self._launch_derivs(small, batched=True))
                     batches += 1
                 progressed = progressed or bool(admitted)
             else:
@@ -253,4 +291,4 @@ class Server:
         return {"mode": mode, "requests": len(reqs), "p50_ms": 1e3 * percentile(lat, 50),
                 "p99_ms": 1e3 * percentile(lat, 99), "mean_ms": 1e3 * float(np.mean(lat)),
                 "throughput_rps": len(reqs) / span if span > 0 else math.inf, "batches": batches,
-                "budget_bytes": self.budget}
+                "streams": len(self.streams), "budget_bytes": self.budget}

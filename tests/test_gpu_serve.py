@@ -18,14 +18,17 @@ def _small_requests(count=24, rate=20000.0, seed=5):
 def test_serve_continuous_completes_and_matches_alone(dev):
     from ntamd.serve import Server
     reqs = _small_requests()
-    srv = Server(keep_results=True, hbm_budget=1 << 30)
+    # fields above 300k points take the fused per-request launch, the rest the
+    # persistent batch: both routes are in the mix
+    srv = Server(keep_results=True, hbm_budget=1 << 30, small_points=300_000)
     stats = srv.run(reqs, mode="continuous")
     assert stats["requests"] == len(reqs)
     assert all(np.isfinite(q.latency) and q.latency >= 0 for q in reqs)
     assert stats["batches"] >= 1
+    assert any(q.kind == "deriv" and not srv._batchable(q) for q in reqs)
     mixed = {q.rid: q.result.clone() for q in reqs}
     for q in reqs:
-        alone = Server(keep_results=True, hbm_budget=1 << 30)
+        alone = Server(keep_results=True, hbm_budget=1 << 30, small_points=300_000)
         alone.run([q], mode="continuous")
         assert torch.equal(q.result, mixed[q.rid]), (q.kind, q.rid)
 
@@ -62,3 +65,12 @@ def test_serve_sequential_baseline(dev):
     reqs = _small_requests(count=8, seed=8)
     stats = Server(hbm_budget=1 << 30).run(reqs, mode="sequential")
     assert stats["requests"] == 8 and stats["batches"] == 0
+
+
+def test_serve_warmup_then_run(dev):
+    from ntamd.serve import Server
+    srv = Server(hbm_budget=1 << 30)
+    srv.warmup()
+    reqs = _small_requests(count=6, seed=9)
+    stats = srv.run(reqs, mode="continuous")
+    assert stats["requests"] == 6 and all(np.isfinite(q.done) for q in reqs)
