@@ -34,8 +34,16 @@ int nt::set_error(int code, const char* fmt, ...) {
 }
 
 namespace {
-// launch-variant table: [layout][precision][ceil(log2 n)]
-int g_tune[2][2][13] = {};
+// launch-variant table: [layout][precision][ceil(log2 n)].  The defaults are
+// the shapes measured fastest on MI355X (compact derivative, about 2^25
+// points): contiguous lines take the wave-private transpose up to 1024 points
+// and direct chunk loads at 2048 (n <= 64 always uses the workgroup tile);
+// strided lines take 32-line tiles except FP64 at 257..512 points, where the
+// 16-line tile with 64 lanes per line is faster.
+int g_tune[2][2][13] = {
+    {{2, 2, 2, 2, 2, 2, 2, 0, 0, 0, 0, 1, 1}, {2, 2, 2, 2, 2, 2, 2, 0, 0, 0, 0, 1, 1}},
+    {{1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1}, {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}},
+};
 
 int log2_bucket(int n) {
   int b = 0;

@@ -136,6 +136,15 @@ __device__ __forceinline__ int opaque_zero() {
 }
 __device__ __forceinline__ void level_fence() { __builtin_amdgcn_sched_barrier(0); }
 
+// Inside a branch on a wave-uniform condition: keeps the branch (an empty
+// volatile asm cannot be speculated), so a patch that applies to one register
+// position is not computed and selected at every position.
+#ifndef NT_NO_KEEP_BRANCH
+#define NT_KEEP_BRANCH() asm volatile("")
+#else
+#define NT_KEEP_BRANCH() ((void)0)
+#endif
+
 // ---- the solve ----------------------------------------------------------------------
 // Solve in place.  `lane` is the lane index inside the segment (0..W-1).
 // Requires n <= W*R and R <= 2^nlev (top rows sit at position R-1 of a lane);
@@ -148,6 +157,18 @@ __device__ __forceinline__ void cr_solve_regs(T (&d)[R], const NtPlanT<T>& P, in
   const bool first_lane = (lane == 0);
   const bool last_lane = (lane == W - 1);
 
+  // Every kept (forward) or eliminated (back) row is first updated as an
+  // interior row; the few rows whose coefficients differ are then patched.
+  // * The first row of each level sits at lane 0, a compile-time position.
+  // * The row that touches the end of the system is the last row (m even) or
+  //   the one before it (m odd, where it absorbs the eliminated last row).
+  //   Its level-0 index is the same for every lane, so its lane and register
+  //   position are scalars.  In the general path a scalar branch per register
+  //   position patches it, instead of a per-row compare and select.
+  // * Rows past the end of the system (general path) are updated too.  They
+  //   stay finite (the padding starts at zero), and no row inside the system
+  //   reads them: the last row's right neighbour enters with factor zero.
+
   // ---- forward reduction, in-lane levels -------------------------------------
 #pragma unroll
   for (int s = 1; s <= LR; ++s) {
@@ -155,35 +176,41 @@ __device__ __forceinline__ void cr_solve_regs(T (&d)[R], const NtPlanT<T>& P, in
     level_fence();
     const NtLevel<T>& Lr = P.lv[s - 1 + opaque_zero()];
     const T k1 = uni(Lr.k1), k2 = uni(Lr.k2), k1f = uni(Lr.k1f), kl = uni(Lr.kl);
-    const int m = FULL ? 0 : uni(Lr.m);
+    // general path: the special row (level-0 index r_sp) and its factors
+    int r_sp = -1;
+    T K1s = k1, K2s = k2;
+    if constexpr (!FULL) {
+      const int m = uni(Lr.m);
+      const bool odd = (m & 1) != 0;
+      r_sp = (((odd ? m - 2 : m - 1) + 1) << (s - 1)) - 1;
+      K1s = odd ? k1 : kl;
+      K2s = odd ? kl : T(0);
+    }
     const T nb = from_next<W>(d[h - 1]);         // row (lane+1)*R + h - 1
 #pragma unroll
     for (int p = (1 << s) - 1; p < R; p += (1 << s)) {
       const T left = d[p - h];
-      const bool has_right_in_lane = (p + h < R);
-      const T right = has_right_in_lane ? d[p + h] : nb;
+      const T right = (p + h < R) ? d[p + h] : nb;
+      T K1 = k1;
+      if (p == (1 << s) - 1) K1 = first_lane ? k1f : K1;
+      T nv;
       if constexpr (FULL) {
-        // only the first kept row of lane 0 and the last row of the last lane are special
-        T K1 = k1;
-        if (p == (1 << s) - 1) K1 = first_lane ? k1f : K1;
-        T nv;
+        // the last row of the system: last lane, last position
         if (p == R - 1) {
           K1 = last_lane ? kl : K1;
           nv = last_lane ? d[p] - K1 * left : d[p] - K1 * left - k2 * right;
         } else {
           nv = d[p] - K1 * left - k2 * right;
         }
-        d[p] = nv;
       } else {
-        const int jp = ((row0 + p + 1) >> (s - 1)) - 1;   // index at level s-1 (odd)
-        const bool valid = jp < m;
-        const bool is_last = (jp == m - 1);
-        const T K1 = (jp == 1) ? k1f : (is_last ? kl : k1);
-        const T K2 = (jp == m - 2) ? kl : k2;
-        const T with_r = d[p] - K1 * left - K2 * right;
-        const T no_r = d[p] - K1 * left;
-        d[p] = valid ? (is_last ? no_r : with_r) : d[p];
+        nv = d[p] - K1 * left - k2 * right;
+        if ((r_sp & (R - 1)) == p) {             // scalar branch
+          NT_KEEP_BRANCH();
+          const T sp = d[p] - K1s * left - K2s * right;
+          nv = (row0 + p == r_sp) ? sp : nv;
+        }
       }
+      d[p] = nv;
     }
   }
 
@@ -198,17 +225,23 @@ __device__ __forceinline__ void cr_solve_regs(T (&d)[R], const NtPlanT<T>& P, in
     level_fence();
     const NtLevel<T>& Lr = P.lv[s - 1 + opaque_zero()];
     const T k1 = uni(Lr.k1), k2 = uni(Lr.k2), k1f = uni(Lr.k1f), kl = uni(Lr.kl);
-    const int m = FULL ? 0 : uni(Lr.m);
+    int l_sp = W - 1;                            // lane of the special row
+    T K1s = kl, K2s = T(0);
+    if constexpr (!FULL) {
+      const int m = uni(Lr.m);
+      const bool odd = (m & 1) != 0;
+      l_sp = (((odd ? m - 2 : m - 1) + 1) << sh) - 1;
+      K1s = odd ? k1 : kl;
+      K2s = odd ? kl : T(0);
+    }
     const T left = shup<W>(v, dl);
     const T right = shdn<W>(v, dl);
-    const int jp = ((lane + 1) >> sh) - 1;
-    const bool kept = (((lane + 1) & ((dl << 1) - 1)) == 0) && (FULL || jp < m);
-    const bool is_last = FULL ? last_lane : (jp == m - 1);
-    const T K1 = (jp == 1) ? k1f : (is_last ? kl : k1);
-    const T K2 = (!FULL && jp == m - 2) ? kl : k2;
-    const T with_r = v - K1 * left - K2 * right;
-    const T no_r = v - K1 * left;
-    v = kept ? (is_last ? no_r : with_r) : v;
+    const bool kept = ((lane + 1) & ((dl << 1) - 1)) == 0;
+    const bool is_sp = (lane == l_sp);
+    const T K1 = (lane == 2 * dl - 1) ? k1f : (is_sp ? K1s : k1);
+    const T K2 = is_sp ? K2s : k2;
+    const T nv = v - K1 * left - K2 * right;
+    v = kept ? nv : v;
   }
 
   // ---- top system (m = 2 or 3 equations at level nlev) ------------------------
@@ -238,6 +271,8 @@ __device__ __forceinline__ void cr_solve_regs(T (&d)[R], const NtPlanT<T>& P, in
   }
 
   // ---- back substitution, cross-lane levels ----------------------------------
+  // interior x = (v - a*xl - c*xr) / b; row 0: (v - cF*xr) / bF; the last row
+  // (m odd, so it is eliminated): (v - aL*xl) / bL
 #pragma unroll
   for (int s = LR + LW; s >= LR + 1; --s) {
     if (!FULL && s > nlev) continue;
@@ -247,22 +282,21 @@ __device__ __forceinline__ void cr_solve_regs(T (&d)[R], const NtPlanT<T>& P, in
     level_fence();
     const NtLevel<T>& Lr = P.lv[s - 1 + opaque_zero()];
     const T a = uni(Lr.a), c = uni(Lr.c), ib = uni(Lr.ib), cf = uni(Lr.cf), ibf = uni(Lr.ibf);
-    const T al = FULL ? T(0) : uni(Lr.al), ibl = FULL ? T(0) : uni(Lr.ibl);
-    const int m = FULL ? 0 : uni(Lr.m);
     const T left = shup<W>(v, dl);
     const T right = shdn<W>(v, dl);
-    const int jp = ((lane + 1) >> sh) - 1;
-    const bool elim = (((lane + 1) & (dl - 1)) == 0) && ((jp & 1) == 0) && (FULL || jp < m);
-    const bool is_first = (jp == 0);
-    const bool is_last = !FULL && (jp == m - 1);
-    // interior x = (v - a*xl - c*xr) / b; row 0: (v - cF*xr) / bF; row m-1 (m odd): (v - aL*xl) / bL
-    const T A = is_last ? al : a;
-    const T C = is_first ? cf : c;
-    const T IB = is_first ? ibf : (is_last ? ibl : ib);
-    const T t_first = (v - C * right) * IB;
-    const T t_last = (v - A * left) * IB;
-    const T t_mid = (v - A * left - C * right) * IB;
-    const T nv = is_first ? t_first : (is_last ? t_last : t_mid);
+    const bool elim = ((lane + 1) & ((dl << 1) - 1)) == dl;   // (lane+1)/dl odd
+    const bool is_first = (lane == dl - 1);
+    T nv = (v - a * left - c * right) * ib;
+    const T t_first = (v - cf * right) * ibf;
+    nv = is_first ? t_first : nv;
+    if constexpr (!FULL) {
+      const int m = uni(Lr.m);
+      if (m & 1) {                               // scalar branch
+        const T al = uni(Lr.al), ibl = uni(Lr.ibl);
+        const T t_last = (v - al * left) * ibl;
+        nv = (lane == (m << sh) - 1) ? t_last : nv;
+      }
+    }
     v = elim ? nv : v;
   }
   d[R - 1] = v;
@@ -275,35 +309,34 @@ __device__ __forceinline__ void cr_solve_regs(T (&d)[R], const NtPlanT<T>& P, in
     level_fence();
     const NtLevel<T>& Lr = P.lv[s - 1 + opaque_zero()];
     const T a = uni(Lr.a), c = uni(Lr.c), ib = uni(Lr.ib), cf = uni(Lr.cf), ibf = uni(Lr.ibf);
-    const T al = FULL ? T(0) : uni(Lr.al), ibl = FULL ? T(0) : uni(Lr.ibl);
-    const int m = FULL ? 0 : uni(Lr.m);
+    // general path: level-0 index of the last row when it is eliminated (m odd)
+    int r_last = -1;
+    T al = T(0), ibl = T(0);
+    if constexpr (!FULL) {
+      const int m = uni(Lr.m);
+      if (m & 1) {
+        r_last = (m << (s - 1)) - 1;
+        al = uni(Lr.al);
+        ibl = uni(Lr.ibl);
+      }
+    }
 #pragma unroll
     for (int p = h - 1; p < R; p += 2 * h) {
       const T left = (p - h >= 0) ? d[p - h] : prev;
       const T right = d[(p + h < R) ? p + h : p];
-      if constexpr (FULL) {
-        // eliminated rows: row 0 of the system only at lane 0, p = h-1; never the last row
-        const T mid = (d[p] - a * left - c * right) * ib;
-        if (p == h - 1) {
-          const T fst = (d[p] - cf * right) * ibf;
-          d[p] = first_lane ? fst : mid;
-        } else {
-          d[p] = mid;
-        }
-      } else {
-        const int jp = ((row0 + p + 1) >> (s - 1)) - 1;   // even
-        const bool valid = jp < m;
-        const bool is_first = (jp == 0);
-        const bool is_last = (jp == m - 1);
-        const T A = is_last ? al : a;
-        const T C = is_first ? cf : c;
-        const T IB = is_first ? ibf : (is_last ? ibl : ib);
-        const T t_first = (d[p] - C * right) * IB;
-        const T t_last = (d[p] - A * left) * IB;
-        const T t_mid = (d[p] - A * left - C * right) * IB;
-        const T nv = is_first ? t_first : (is_last ? t_last : t_mid);
-        d[p] = valid ? nv : d[p];
+      T nv = (d[p] - a * left - c * right) * ib;
+      if (p == h - 1) {                          // row 0 of the system: lane 0
+        const T fst = (d[p] - cf * right) * ibf;
+        nv = first_lane ? fst : nv;
       }
+      if constexpr (!FULL) {
+        if (r_last >= 0 && (r_last & (R - 1)) == p) {   // scalar branch
+          NT_KEEP_BRANCH();
+          const T lst = (d[p] - al * left) * ibl;
+          nv = (row0 + p == r_last) ? lst : nv;
+        }
+      }
+      d[p] = nv;
     }
   }
 }

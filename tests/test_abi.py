@@ -122,3 +122,28 @@ def test_dropin_constructor_checks():
         NearToeplitzSolver(4096, 4, COEFFS["compact"])
     s = NearToeplitzSolver(96, 10, COEFFS["compact"])   # non power of two is accepted
     assert s.plan.nlevels == len(cr.level_systems(96, COEFFS["compact"]))
+
+
+def test_tuning_table_set_get_and_validation():
+    """The launch-shape table is host state: defaults are in range, set/get
+    round-trips per (layout, precision, n bucket), bad arguments are rejected."""
+    import ctypes
+    from ntamd import _lib
+    lib = _lib.load()
+    v, nv = ctypes.c_int(), ctypes.c_int()
+    for layout in (0, 1):
+        for prec in (_lib.NT_PREC_F64, _lib.NT_PREC_F32_REFINE):
+            for n in (2, 64, 100, 512, 2048):
+                assert lib.nt_tune_get(layout, prec, n, ctypes.byref(v), ctypes.byref(nv)) == 0
+                assert 0 <= v.value < nv.value
+    keep = ctypes.c_int()
+    assert lib.nt_tune_get(1, 1, 700, ctypes.byref(keep), None) == 0
+    assert lib.nt_tune_set(1, 1, 700, 1 - keep.value) == 0
+    assert lib.nt_tune_get(1, 1, 1024, ctypes.byref(v), None) == 0 and v.value == 1 - keep.value  # same bucket
+    assert lib.nt_tune_get(1, 0, 700, ctypes.byref(v), None) == 0                                # other precision
+    assert lib.nt_tune_set(1, 1, 700, keep.value) == 0
+    assert lib.nt_tune_set(2, 0, 64, 0) == _lib.NT_EINVAL
+    assert lib.nt_tune_set(0, 5, 64, 0) == _lib.NT_EINVAL
+    assert lib.nt_tune_set(0, 0, 1, 0) == _lib.NT_EINVAL
+    assert lib.nt_tune_set(0, 0, 64, 3) == _lib.NT_EINVAL
+    assert b"variant" in lib.nt_last_error()
