@@ -6,11 +6,14 @@ f = sin(2 pi x) sin(2 pi y) sin(2 pi z) + N(0, 1e-3) noise (seed 0), i.e.
 stencil + cyclic reduction, each point read once and written once).
 Metric: grid points per second and the fraction of HBM peak.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick] [--case NAME ...]
 
-Prints ONE JSON line.  Other configurations (d/dy, d/dz, FP32 + refinement,
-FP32 storage at 1024^3, ADI 2048^2 x 200 steps in a hipGraph) are timed after
-the headline and reported under "extras".
+Prints ONE JSON line.  Other configurations are timed after the headline and
+reported under "extras": d/dy and d/dz ("axes"), FP32 + refinement sweeps
+("refine"), ADI 2048^2 x 200 steps in a hipGraph ("adi"), a 1024^3 field
+stored in FP32 ("fp32_storage"), 64 mixed-size grids in one persistent launch
+("batch"), and a Poisson request stream served with continuous batching
+("serving").  --quick skips them; --case runs only the named ones.
 """
 import argparse
 import json
@@ -212,6 +215,13 @@ def mixed_batch_case(count=64, seed=1, reps=3):
         pts += int(np.prod(shape))
     b = MixedBatch(jobs)
     t = time_launches(b.run, reps, 1)
+    plans = {n: Plan(n, COMPACT_COEFFS) for n in {j["f"].shape[j["axis"]] for j in jobs}}
+
+    def separate():
+        for j in jobs:
+            core.compact_derivative(plans[j["f"].shape[j["axis"]]], j["f"], j["out"], j["dx"], j["axis"],
+                                    j["precision"], j["sweeps"])
+    ts = time_launches(separate, reps, 1)
     errs = []
     for j in jobs:
         ref = torch.empty_like(j["out"])
@@ -219,10 +229,12 @@ def mixed_batch_case(count=64, seed=1, reps=3):
         core.compact_derivative(Plan(n, COMPACT_COEFFS), j["f"], ref, j["dx"], j["axis"], "fp64", 1)
         errs.append(((j["out"] - ref).abs().max() / ref.abs().max()).item())
     res = {"grids": count, "points": pts, "launch_s": t["mean_s"], "pts_per_s": pts / t["mean_s"],
-           "gbps": pts * 16 / t["mean_s"] / 1e9, "max_rel_err_vs_fp64_per_grid_max": max(errs),
+           "gbps": pts * 16 / t["mean_s"] / 1e9, "separate_launches_s": ts["mean_s"],
+           "max_rel_err_vs_fp64_per_grid_max": max(errs),
            "max_rel_err_vs_fp64_per_grid": errs}
-    log(f"[bench] mixed batch: {count} grids, {pts / 1e6:.0f} M pts, {t['mean_s'] * 1e3:.3f} ms, "
-        f"{res['gbps']:.0f} GB/s, worst grid err {max(errs):.2e}")
+    log(f"[bench] mixed batch: {count} grids, {pts / 1e6:.0f} M pts, {t['mean_s'] * 1e3:.3f} ms "
+        f"({res['gbps']:.0f} GB/s; {count} separate launches {ts['mean_s'] * 1e3:.3f} ms), "
+        f"worst grid err {max(errs):.2e}")
     del jobs, b
     torch.cuda.empty_cache()
     return res
@@ -250,6 +262,9 @@ def serving_case(count=1000, rates=(500.0, 2000.0), seed=2):
     return out
 
 
+EXTRA_CASES = ("axes", "refine", "adi", "fp32_storage", "batch", "serving")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,10 +274,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short program profiled for HBM traffic and exit")
+    ap.add_argument("--case", action="append", choices=EXTRA_CASES,
+                    help="run only these extra cases after the headline (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(lines shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a derivative's lines "
+              "shard across GPUs without communication; run one process per GPU on its share)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -283,25 +301,31 @@ def main():
     log(f"[bench] d/dx: {head['mean_s'] * 1e6:.1f} us/launch, {head['gbps']:.0f} GB/s")
 
     extras = {}
-    if not args.quick:
-        short = max(10, args.steps // 2)
+    cases = set(args.case) if args.case else (set() if args.quick else set(EXTRA_CASES))
+    short = max(10, args.steps // 2)
+    if "axes" in cases:
         for name, axis in (("ddy_fp64", 1), ("ddz_fp64", 0)):
             t, _, _ = compact_case(N, axis, "fp64", 1, torch.float64, short, 3, field=f64)
             extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
             log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, {t['gbps']:.0f} GB/s")
+    if "refine" in cases:
         for name, sweeps in (("ddx_fp32_refine1", 1), ("ddx_fp32_refine2", 2)):
             t, _, o2 = compact_case(N, 2, "fp32_refine", sweeps, torch.float64, short, 3, field=f64)
             err = ((o2 - out).abs().max() / out.abs().max()).item()
             extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
             extras[name]["max_rel_err_vs_fp64"] = err
             log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, err {err:.2e}")
-        del out, f64
-        torch.cuda.empty_cache()
+    del out, f64
+    torch.cuda.empty_cache()
+    if "adi" in cases:
         a = adi_case(2048, 0.5, 200, 3)
         extras["adi_2048_200steps_graph"] = a
         log(f"[bench] ADI 2048^2: {a['ms_per_step']:.3f} ms/step")
+    if "fp32_storage" in cases:
         extras["fp32_storage_1024"] = config4_case()
+    if "batch" in cases:
         extras["mixed_batch_64"] = mixed_batch_case()
+    if "serving" in cases:
         extras["serving"] = serving_case()
 
     cpu = None
