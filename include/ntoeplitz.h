@@ -147,6 +147,30 @@ int nt_part_plan_secondary(const nt_part_plan* plan, int p, double* xU, double* 
 int nt_part_solve_lines(const nt_part_plan* plan, int producer, const void* in, void* out,
                         int64_t lines, double p1, double p2, void* stream);
 
+/* The partition method one step at a time, as the reference's ranks run it
+ * (run-parallel-compact.py:72-76).  Subdomain arrays are FP64, C-contiguous.
+ *
+ * parallelcompact.rhs.compute_RHS(f_local, x, dx, shape, mx, npx)
+ * (parallelcompact/rhs.py:8-73): x (nz, ny, nx) = compact RHS along x of the
+ * width-1 ghosted f_local (nz+2, ny+2, nx+2); the one-sided closures only on
+ * rank mx == 0 (first point) and mx == npx-1 (last point).  nx >= 3. */
+int nt_part_rhs_ghosted(const void* f_local, void* x, int64_t nz, int64_t ny, int64_t nx, double dx,
+                        int mx, int npx, void* stream);
+
+/* parallelcompact.reduced.get_params(line_da, xU_d, xL_d, xR_d)
+ * (parallelcompact/reduced.py:135-196) for the `nsub` subdomains of a line
+ * held on one device, so the reference's gather / root solve / scatter
+ * (reduced.py:147-192) is one launch: xU, xL (nsub, m) secondary solutions,
+ * xR (nsub, nz, ny, m) subdomain solutions; writes alpha, beta (nsub, nz, ny).
+ * 1 <= nsub <= 16. */
+int nt_part_params(const void* xU, const void* xL, const void* xR, void* alpha, void* beta, int nsub,
+                   int64_t nz, int64_t ny, int64_t m, void* stream);
+
+/* parallelcompact.sum.sum_solutions(x_UH, x_LH, x_R, alpha, beta, shape)
+ * (parallelcompact/sum.py:7-51): x_R[z,y,x] += alpha[z,y] x_UH[x] + beta[z,y] x_LH[x]. */
+int nt_part_sum(const void* xU, const void* xL, void* xR, const void* alpha, const void* beta, int64_t nz,
+                int64_t ny, int64_t nx, void* stream);
+
 /* ---- mixed-size persistent batch --------------------------------------------
  * No reference counterpart (the reference launches one grid per field and
  * direction, run-compact.py:57-90).  Many fields of different extents,

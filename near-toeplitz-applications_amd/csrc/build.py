@@ -29,7 +29,7 @@ SOURCES = [
     "nt_k_compact32_strided.hip",
     "nt_k_adi_contig.hip",
     "nt_k_adi_strided.hip",
-    "nt_part.hip",
+    "nt_part.hip", "nt_part_ops.hip",
     "nt_batch.hip",
 ]
 

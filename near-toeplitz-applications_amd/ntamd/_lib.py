@@ -55,6 +55,9 @@ SIGNATURES = {
     "nt_part_plan_destroy": (_int, [_vp]),
     "nt_part_plan_secondary": (_int, [_vp, _int, _dp, _dp, _int]),
     "nt_part_solve_lines": (_int, [_vp, _int, _vp, _vp, _i64, _dbl, _dbl, _vp]),
+    "nt_part_rhs_ghosted": (_int, [_vp, _vp, _i64, _i64, _i64, _dbl, _int, _int, _vp]),
+    "nt_part_params": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _vp]),
+    "nt_part_sum": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),
     "nt_batch_image_bytes": (ctypes.c_size_t, [_int, _int]),
     "nt_batch_pack": (_int, [_vp, _int, _pp, _int, _vp, ctypes.c_size_t, _i64p]),
     "nt_batch_launch": (_int, [_vp, _int, _int, _i64, _vp, _int, _vp]),

@@ -212,3 +212,49 @@ def adi_steps(plan, u, work, r, nsteps, precision="fp64", sweeps=1, stream=None)
     _lib.call("nt_adi_steps", plan.handle, device_ptr(u, "u"), device_ptr(work, "work"), N, float(r),
               int(nsteps), precision_code(precision), int(sweeps), stream_handle(stream))
     return u
+
+
+# ---- partition method, one step at a time (parallelcompact/rhs.py, reduced.py, sum.py) ----
+
+def part_rhs_ghosted(f_local, x, dx, mx, npx, stream=None):
+    """Compact RHS along x of a width-1 ghosted subdomain array:
+    f_local (nz+2, ny+2, nx+2) -> x (nz, ny, nx); closures on rank 0 / npx-1."""
+    fp = device_ptr(f_local, "f_local")
+    xp = device_ptr(x, "x")
+    nz, ny, nx = _shape3(x, "x")
+    if tuple(f_local.shape) != (nz + 2, ny + 2, nx + 2):
+        raise NtInvalidArgument(_lib.NT_EINVAL,
+                                f"f_local shape {tuple(f_local.shape)} != {(nz + 2, ny + 2, nx + 2)}")
+    _lib.call("nt_part_rhs_ghosted", fp, xp, nz, ny, nx, float(dx), int(mx), int(npx), stream_handle(stream))
+    return x
+
+
+def part_params(xU, xL, xR, alpha, beta, stream=None):
+    """alpha, beta (P, nz, ny) of the reduced system for P subdomains on this
+    device: xU, xL (P, m) secondary solutions, xR (P, nz, ny, m)."""
+    up, lp, rp = device_ptr(xU, "xU"), device_ptr(xL, "xL"), device_ptr(xR, "xR")
+    ap_, bp = device_ptr(alpha, "alpha"), device_ptr(beta, "beta")
+    if xR.dim() != 4:
+        raise NtInvalidArgument(_lib.NT_EINVAL, f"xR must be (P, nz, ny, m), got shape {tuple(xR.shape)}")
+    P, nz, ny, m = (int(s) for s in xR.shape)
+    for t, name in ((xU, "xU"), (xL, "xL")):
+        if tuple(t.shape) != (P, m):
+            raise NtInvalidArgument(_lib.NT_EINVAL, f"{name} shape {tuple(t.shape)} != {(P, m)}")
+    for t, name in ((alpha, "alpha"), (beta, "beta")):
+        if tuple(t.shape) != (P, nz, ny):
+            raise NtInvalidArgument(_lib.NT_EINVAL, f"{name} shape {tuple(t.shape)} != {(P, nz, ny)}")
+    _lib.call("nt_part_params", up, lp, rp, ap_, bp, P, nz, ny, m, stream_handle(stream))
+    return alpha, beta
+
+
+def part_sum(xU, xL, xR, alpha, beta, stream=None):
+    """xR (nz, ny, nx) += alpha[z, y] xU[x] + beta[z, y] xL[x], in place."""
+    up, lp, rp = device_ptr(xU, "x_UH"), device_ptr(xL, "x_LH"), device_ptr(xR, "x_R")
+    ap_, bp = device_ptr(alpha, "alpha"), device_ptr(beta, "beta")
+    nz, ny, nx = _shape3(xR, "x_R")
+    if xU.numel() != nx or xL.numel() != nx:
+        raise NtInvalidArgument(_lib.NT_EINVAL, f"x_UH and x_LH must hold nx = {nx} values")
+    if alpha.numel() != nz * ny or beta.numel() != nz * ny:
+        raise NtInvalidArgument(_lib.NT_EINVAL, f"alpha and beta must hold nz*ny = {nz * ny} values")
+    _lib.call("nt_part_sum", up, lp, rp, ap_, bp, nz, ny, nx, stream_handle(stream))
+    return xR
