@@ -3,7 +3,7 @@
 // Plan time (host, FP64, once): the three subdomain level tables, the
 // secondary solutions xU/xL of parallelcompact/misc.py:21-46 (here by a Thomas
 // sweep on the host instead of SciPy), the reduced system of
-// parallelcompact/reduced.py:163-178 and its Thomas factors.  Everything is
+// parallelcompact/reduced.py:163-178 and its inverse.  Everything is
 // copied to one device allocation owned by the plan; the reference instead
 // recomputes the secondary solutions and reallocates buffers on every call
 // (run-parallel-compact.py:82, reduced.py:136-145).
@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -130,17 +131,16 @@ int nt_part_plan_create(int64_t n, int P, const double coeffs[7], nt_part_plan**
   rb[N2 - 1] = 1.0;
   ra[1] = 0.0;
   rc[N2 - 2] = 0.0;
-  double c2 = 0.0;
-  for (int i = 0; i < N2; ++i) {
-    const double den = rb[i] - (i > 0 ? ra[i] * c2 : 0.0);
-    if (den == 0.0 || !std::isfinite(den)) {
+  // its inverse, column by column with the reference's Thomas sweep (reduced.py:11-49)
+  std::vector<double> ej(N2), col(N2);
+  for (int j = 0; j < N2; ++j) {
+    std::fill(ej.begin(), ej.end(), 0.0);
+    ej[j] = 1.0;
+    if (!thomas(N2, ra.data(), rb.data(), rc.data(), ej.data(), col.data())) {
       delete pp;
       return nt::set_error(NT_ESINGULAR, "%s", "nt_part_plan_create: zero pivot in the reduced system");
     }
-    pp->host.red_a[i] = ra[i];
-    pp->host.red_ib[i] = 1.0 / den;
-    c2 = rc[i] / den;
-    pp->host.red_c2[i] = c2;
+    for (int i = 0; i < N2; ++i) pp->host.red_inv[i * N2 + j] = col[i];
   }
   pp->host.m = m;
   pp->host.P = P;

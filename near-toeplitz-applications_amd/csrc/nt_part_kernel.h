@@ -12,9 +12,11 @@
 // through LDS; form the RHS in line coordinates (the "ghost" values of
 // parallelcompact/rhs.py are simply the neighbouring subdomain's values);
 // solve each subdomain in registers; put the negated end values of every
-// subdomain in LDS; one lane runs the reduced system's Thomas sweep with
-// factors precomputed at plan time; every subdomain adds
-// alpha*xU + beta*xL and the line goes back to HBM.
+// subdomain in LDS; solve the reduced system by multiplying with its inverse
+// (the same for every line, computed at plan time), one output per thread, so
+// no thread walks a serial sweep; every subdomain adds alpha*xU + beta*xL and
+// the line goes back to HBM.  Each wavefront stages and stores its own
+// subdomain, so no LDS index needs a division by m.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,10 +35,8 @@ struct NtPartDev {
   int32_t nlev;                     // reductions of an m-point subdomain
   int32_t m, P, n;
   double b1, c1, ai, bi, ci, an, bn;
-  // reduced system (2P unknowns): a[i] sub-diagonal, ib[i] = 1/(b[i] - a[i] c2[i-1]), c2[i]
-  double red_a[2 * NT_PART_MAX_P];
-  double red_ib[2 * NT_PART_MAX_P];
-  double red_c2[2 * NT_PART_MAX_P];
+  // inverse of the reduced system (2P unknowns), row-major with row stride 2P
+  double red_inv[4 * NT_PART_MAX_P * NT_PART_MAX_P];
   // followed in the same allocation by xU[3][m] and xL[3][m]
 };
 
@@ -56,15 +56,16 @@ __global__ void __launch_bounds__(64 * NT_PART_MAX_P) part_kernel(const PartArgs
   constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
   constexpr int ROWS = 1 + 2 * HALO;
   constexpr int CP = (R >= 4) ? 2 : 0;
+  constexpr int LR = Log2<R>::value;
   // LDS layout of a staged line: subdomain q, element el -> q*SUBP + el + (el/R)*CP
   constexpr int LMAX = NT_PART_MAX_N + (NT_PART_MAX_N / R) * CP + NT_PART_MAX_P * (R + CP + 2);
   __shared__ double lds[ROWS * LMAX];
   __shared__ double red[2 * NT_PART_MAX_P];
+  __shared__ double sol[2 * NT_PART_MAX_P];
 
   const NtPartDev& D = *args.dev;
   const int m = D.m, P = D.P, n = D.n;
   const int tid = threadIdx.x;
-  const int nthreads = blockDim.x;
   const int p = __builtin_amdgcn_readfirstlane(tid >> 6);     // my subdomain (wave-uniform)
   const int lane = tid & 63;
   const int kind = (P == 1) ? 0 : (p == 0 ? 0 : (p == P - 1 ? 2 : 1));
@@ -72,42 +73,55 @@ __global__ void __launch_bounds__(64 * NT_PART_MAX_P) part_kernel(const PartArgs
   const int64_t line = static_cast<int64_t>(blockIdx.x) + HALO;   // staged line index of the system
   const int SUBP = ((m + R - 1) / R) * (R + CP) + 2;             // LDS stride between subdomains
   const int LSTR = P * SUBP;                                       // LDS stride between staged lines
+  auto at = [&](int row, int q, int el) -> int { return row * LSTR + q * SUBP + el + (el >> LR) * CP; };
 
-  auto lds_at = [&](int row, int e) -> int {     // e: position along the whole line
-    const int q = e / m, el = e - q * m;
-    return row * LSTR + q * SUBP + el + (el / R) * CP;
-  };
-
-  // ---- stage the line (+ halo lines) -------------------------------------------
+  // ---- stage: wavefront p copies its own subdomain of each staged line -----------
+  // (m <= 64 R, so R loads per lane cover it; all are issued before the LDS writes)
   const double* __restrict__ in = static_cast<const double*>(args.in);
-  for (int idx = tid; idx < ROWS * n; idx += nthreads) {
-    const int r = idx / n, e = idx - r * n;
-    lds[lds_at(r, e)] = in[(line - HALO + r) * args.lstride + e];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    const double* __restrict__ src = in + (line - HALO + r) * args.lstride + p * m;
+    double v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int el = lane + 64 * k;
+      v[k] = (el < m) ? __builtin_nontemporal_load(src + el) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int el = lane + 64 * k;
+      if (el < m) lds[at(r, p, el)] = v[k];
+    }
   }
   __syncthreads();
 
-  // ---- right-hand side in line coordinates ------------------------------------------
+  // ---- right-hand side in line coordinates: the neighbouring subdomain's values
+  // are the ghost points of parallelcompact/rhs.py -------------------------------------
   const int my_el0 = lane * R;          // first row of my chunk inside the subdomain
   double d[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int el = my_el0 + k;
-    const int e = p * m + el;
     double v = 0.0;
     if (el < m) {
-      const double c = lds[lds_at(HALO, e)];
+      const double c = lds[at(HALO, p, el)];
       if (PROD == P_COMPACT) {
+        const int e = p * m + el;
         if (e == 0) {
-          v = args.s2 * (-5.0 * c + 4.0 * lds[lds_at(0, 1)] + lds[lds_at(0, 2)]);
+          v = args.s2 * (-5.0 * c + 4.0 * lds[at(0, 0, 1)] + lds[at(0, 0, 2)]);
         } else if (e == n - 1) {
-          v = -args.s2 * (-5.0 * c + 4.0 * lds[lds_at(0, n - 2)] + lds[lds_at(0, n - 3)]);
+          v = -args.s2 * (-5.0 * c + 4.0 * lds[at(0, P - 1, m - 2)] + lds[at(0, P - 1, m - 3)]);
         } else {
-          v = args.s1 * (lds[lds_at(0, e + 1)] - lds[lds_at(0, e - 1)]);
+          // neighbours across a subdomain edge (never past the line's ends: e == 0
+          // and e == n-1 took the closures above; the clamps keep every index in range)
+          const int pl = (p > 0) ? p - 1 : 0, pr = (p < P - 1) ? p + 1 : P - 1;
+          const double fl = (el > 0) ? lds[at(0, p, el - 1)] : lds[at(0, pl, m - 1)];
+          const double fr = (el < m - 1) ? lds[at(0, p, el + 1)] : lds[at(0, pr, 0)];
+          v = args.s1 * (fr - fl);
         }
       } else if (PROD == P_ADI) {
-        v = (e == 0 || e == n - 1)
-                ? c
-                : args.s1 * (lds[lds_at(0, e)] + lds[lds_at(2, e)]) + args.s2 * c;
+        const int e = p * m + el;
+        v = (e == 0 || e == n - 1) ? c : args.s1 * (lds[at(0, p, el)] + lds[at(2, p, el)]) + args.s2 * c;
       } else {
         v = c;
       }
@@ -129,35 +143,32 @@ __global__ void __launch_bounds__(64 * NT_PART_MAX_P) part_kernel(const PartArgs
   }
   __syncthreads();
 
-  // ---- reduced system: serial Thomas sweep with plan-time factors (reduced.py:11-49)
-  if (tid == 0) {
-    const int N2 = 2 * P;
-    double y = red[0] * D.red_ib[0];
-    red[0] = y;
-    for (int i = 1; i < N2; ++i) {
-      y = (red[i] - D.red_a[i] * y) * D.red_ib[i];
-      red[i] = y;
-    }
-    double x = red[N2 - 1];
-    for (int i = N2 - 2; i >= 0; --i) {
-      x = red[i] - D.red_c2[i] * x;
-      red[i] = x;
-    }
+  // ---- reduced system: (alpha_0, beta_0, alpha_1, ...) = inverse * faces -----------
+  const int N2 = 2 * P;
+  if (tid < N2) {
+    const double* __restrict__ row = D.red_inv + tid * N2;
+    double u = 0.0;
+    for (int j = 0; j < N2; ++j) u += row[j] * red[j];
+    sol[tid] = u;
   }
   __syncthreads();
 
   // ---- correction x = xR + alpha xU + beta xL (sum.py:30) and store -----------------
-  const double alpha = red[2 * p], beta = red[2 * p + 1];
+  const double alpha = sol[2 * p], beta = sol[2 * p + 1];
   const double* __restrict__ xu = args.xu + kind * m;
   const double* __restrict__ xl = args.xl + kind * m;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int el = my_el0 + k;
-    if (el < m) lds[lds_at(HALO, p * m + el)] = d[k] + alpha * xu[el] + beta * xl[el];
+    if (el < m) lds[at(HALO, p, el)] = d[k] + alpha * xu[el] + beta * xl[el];
   }
-  __syncthreads();
-  double* __restrict__ out = static_cast<double*>(args.out);
-  for (int e = tid; e < n; e += nthreads) out[line * args.lstride + e] = lds[lds_at(HALO, e)];
+  wave_lds_sync();                       // this wavefront reads back only its own subdomain
+  double* __restrict__ dst = static_cast<double*>(args.out) + line * args.lstride + p * m;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int el = lane + 64 * k;
+    if (el < m) __builtin_nontemporal_store(lds[at(HALO, p, el)], dst + el);
+  }
 }
 
 }  // namespace nt
