@@ -194,24 +194,25 @@ def config4_case(N=1024, reps=5):
     return res
 
 
-def mixed_batch_case(count=64, seed=1, reps=3):
+def mixed_batch_case(count=64, seed=1, reps=3, sizes=None, precision="fp32_refine"):
     """64 grids, per-axis sizes uniform in {64, 96, ..., 512} (seeded), random
-    derivative direction each, FP32 solve + one sweep, all in one persistent launch.
-    Per-grid max relative error against the plain FP64 kernel."""
+    derivative direction each, FP32 solve + one sweep, all in one persistent launch,
+    against the same jobs as separate fused launches.  Per-grid max relative
+    error against the plain FP64 kernel."""
     import numpy as np
     import torch
     from ntamd import COMPACT_COEFFS, Plan, core
     from ntamd.batch import MixedBatch
     from ntamd.workloads import compact_field_device
     rng = np.random.default_rng(seed)
-    sizes = np.arange(64, 513, 32)
+    sizes = np.arange(64, 513, 32) if sizes is None else np.asarray(sizes)
     jobs, pts = [], 0
     for i in range(count):
         shape = tuple(int(s) for s in rng.choice(sizes, size=3))
         axis = int(rng.integers(0, 3))
         f = compact_field_device(shape, seed=100 + i)
         jobs.append(dict(f=f, out=torch.empty_like(f), axis=axis, dx=1.0 / (shape[axis] - 1),
-                         precision="fp32_refine", sweeps=1))
+                         precision=precision, sweeps=1))
         pts += int(np.prod(shape))
     b = MixedBatch(jobs)
     t = time_launches(b.run, reps, 1)
@@ -283,6 +284,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     from ntamd import _lib, autotune
     _lib.load()
@@ -325,6 +327,7 @@ def main():
         extras["fp32_storage_1024"] = config4_case()
     if "batch" in cases:
         extras["mixed_batch_64"] = mixed_batch_case()
+        extras["mixed_batch_small_1024"] = mixed_batch_case(count=1024, sizes=np.arange(16, 65, 8), reps=5)
     if "serving" in cases:
         extras["serving"] = serving_case()
 

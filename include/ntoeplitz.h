@@ -44,9 +44,9 @@ int nt_version(void);
 
 /* ---- launch tuning (no reference counterpart: the reference hard-codes one
  * block per system, neato.py:181-183) --------------------------------------
- * Launch-shape variant used for lines of length n (bucketed by ceil(log2 n))
- * in layout 0 (contiguous lines) or 1 (strided lines) at the given precision
- * mode (NT_PREC_*).  Process-wide; the defaults are the shapes measured
+ * Launch-shape variant used for lines of length n (bucketed by ceil(log2 n)
+ * and by whether n is a power of two) in layout 0 (contiguous lines) or 1
+ * (strided lines) at the given precision mode (NT_PREC_*).  Process-wide; the defaults are the shapes measured
  * fastest on MI355X, and the autotuner (ntamd/autotune.py) can re-measure and
  * set them from its JSON cache.  Every variant gives the same results to
  * within one rounding; only the launch shape changes. */

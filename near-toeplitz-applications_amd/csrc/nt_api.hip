@@ -34,16 +34,21 @@ int nt::set_error(int code, const char* fmt, ...) {
 }
 
 namespace {
-// launch-variant table: [layout][precision][ceil(log2 n)].  The defaults are
-// the shapes measured fastest on MI355X (compact derivative, about 2^25
-// points): contiguous lines take the wave-private transpose up to 1024 points
-// and direct chunk loads at 2048 (n <= 64 always uses the workgroup tile);
-// strided lines take 32-line tiles except FP64 at 257..512 points, where the
-// 16-line tile with 64 lanes per line is faster.
-int g_tune[2][2][13] = {
-    {{2, 2, 2, 2, 2, 2, 2, 0, 0, 0, 0, 1, 1}, {2, 2, 2, 2, 2, 2, 2, 0, 0, 0, 0, 1, 1}},
-    {{1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1}, {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}},
+// launch-variant table: [layout][precision][ceil(log2 n)][n is a power of two].
+// Powers of two take the kernels' compile-time level structure, other lengths
+// the general path, and the two prefer different shapes.  The defaults are the
+// shapes measured fastest on MI355X (compact derivative, about 2^25 points,
+// the other two extents multiples of 64, n = 2^k and n = 3 * 2^(k-2)).
+int g_tune[2][2][13][2] = {
+    // contiguous lines: 0 wave-private transpose, 1 direct chunk loads, 2 workgroup tile
+    {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 1}, {1, 1}},
+     {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 1}, {0, 1}}},
+    // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512)
+    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 0}, {0, 0}, {1, 0}, {1, 1}, {1, 1}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 0}, {1, 1}, {0, 0}, {1, 1}, {1, 1}}},
 };
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 int log2_bucket(int n) {
   int b = 0;
@@ -52,9 +57,11 @@ int log2_bucket(int n) {
 }
 }  // namespace
 
-int nt::tune_get(int layout, int prec, int n) { return g_tune[layout & 1][prec & 1][log2_bucket(n)]; }
+int nt::tune_get(int layout, int prec, int n) {
+  return g_tune[layout & 1][prec & 1][log2_bucket(n)][is_pow2(n) ? 1 : 0];
+}
 void nt::tune_set(int layout, int prec, int n, int variant) {
-  g_tune[layout & 1][prec & 1][log2_bucket(n)] = variant;
+  g_tune[layout & 1][prec & 1][log2_bucket(n)][is_pow2(n) ? 1 : 0] = variant;
 }
 
 namespace {

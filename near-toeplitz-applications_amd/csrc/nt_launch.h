@@ -14,7 +14,8 @@ namespace nt {
 template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
 hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 
-// Launch-shape variants, chosen per (layout, precision, ceil(log2 n)) by the tuning table
+// Launch-shape variants, chosen per (layout, precision, ceil(log2 n), n a power of two)
+// by the tuning table
 // (nt_tune_set; defaults measured on MI355X).  Every variant computes the same values.
 //   CONTIG  0: one line per wavefront, wave-coalesced nontemporal loads and
 //              stores, wave-private LDS transpose (needs 16-byte aligned lines)

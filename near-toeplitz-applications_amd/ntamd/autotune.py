@@ -1,7 +1,8 @@
 """Launch-shape autotuner for the line kernels.
 
-For each (layout, precision mode, line length n) the library has several equivalent launch
-shapes (``nt_tune_set``): contiguous lines can stream through a wave-private
+For each layout, precision mode and line-length bucket (powers of two in a
+slot of their own) the library has several equivalent launch shapes
+(``nt_tune_set``): contiguous lines can stream through a wave-private
 LDS transpose, load per-lane chunks directly, or go through a workgroup tile;
 strided lines can use tiles of 16 or of 32 adjacent lines.  ``tune()`` times
 every variant on a synthetic field of the given shape class, keeps the fastest,
@@ -58,30 +59,36 @@ def _time(fn, reps):
 def tune(ns=(64, 128, 256, 512, 1024, 2048), points=1 << 25, reps=10, precisions=("fp64", "fp32_refine"),
          verbose=False):
     """Time every variant for contiguous (layout 0) and strided (layout 1) lines
-    of each length in ``ns``, at each precision mode, on a field of about
-    ``points`` points; apply and return
-    ``{"<layout>:<precision>:<n>": {"variant": v, "seconds": [...]}}``."""
+    at each precision mode, on fields of about ``points`` points, and apply the
+    fastest.  The table has two slots per length bucket: powers of two (timed at
+    n, where the levels are known at compile time) and other lengths (timed at
+    3n/4, the general path).  Returns
+    ``{"<layout>:<precision>:<n>": {"variant": v, "seconds": [...]}}`` for both."""
     dev = torch.device("cuda")
     result = {}
     for n in ns:
-        other = max(1, int((points / n) ** 0.5))
-        plan = Plan(n, COMPACT)
-        for layout, axis, shape in ((0, 2, (other, other, n)), (1, 1, (other, n, other))):
-            f = torch.randn(shape, dtype=torch.float64, device=dev)
-            out = torch.empty_like(f)
-            for prec in precisions:
-                _, nv = current(layout, n, prec)
-                times = []
-                for v in range(nv):
-                    _set(layout, prec, n, v)
-                    times.append(_time(lambda: compact_derivative(plan, f, out, 1.0 / (n - 1), axis, prec), reps))
-                best = min(range(nv), key=lambda i: times[i])
-                _set(layout, prec, n, best)
-                result[f"{layout}:{prec}:{n}"] = {"variant": best, "seconds": times}
-                if verbose:
-                    print(f"layout {layout} {prec} n {n}: " +
-                          ", ".join(f"v{i} {t * 1e6:.1f}us" for i, t in enumerate(times)), flush=True)
-            del f, out
+        for length in ([n] if n < 16 else [n, 3 * n // 4]):
+            # the other two extents: multiples of 64, as in real grids (aligned
+            # rows let the strided kernels use 16-byte row loads)
+            other = max(64, int((points / length) ** 0.5) // 64 * 64)
+            plan = Plan(length, COMPACT)
+            for layout, axis, shape in ((0, 2, (other, other, length)), (1, 1, (other, length, other))):
+                f = torch.randn(shape, dtype=torch.float64, device=dev)
+                out = torch.empty_like(f)
+                for prec in precisions:
+                    _, nv = current(layout, length, prec)
+                    times = []
+                    for v in range(nv):
+                        _set(layout, prec, length, v)
+                        times.append(_time(lambda: compact_derivative(plan, f, out, 1.0 / (length - 1), axis, prec),
+                                           reps))
+                    best = min(range(nv), key=lambda i: times[i])
+                    _set(layout, prec, length, best)
+                    result[f"{layout}:{prec}:{length}"] = {"variant": best, "seconds": times}
+                    if verbose:
+                        print(f"layout {layout} {prec} n {length}: " +
+                              ", ".join(f"v{i} {t * 1e6:.1f}us" for i, t in enumerate(times)), flush=True)
+                del f, out
     return result
 
 

@@ -138,9 +138,11 @@ def test_tuning_table_set_get_and_validation():
                 assert 0 <= v.value < nv.value
     keep = ctypes.c_int()
     assert lib.nt_tune_get(1, 1, 700, ctypes.byref(keep), None) == 0
+    p2 = ctypes.c_int()
+    assert lib.nt_tune_get(1, 1, 1024, ctypes.byref(p2), None) == 0
     assert lib.nt_tune_set(1, 1, 700, 1 - keep.value) == 0
-    assert lib.nt_tune_get(1, 1, 1024, ctypes.byref(v), None) == 0 and v.value == 1 - keep.value  # same bucket
-    assert lib.nt_tune_get(1, 0, 700, ctypes.byref(v), None) == 0                                # other precision
+    assert lib.nt_tune_get(1, 1, 900, ctypes.byref(v), None) == 0 and v.value == 1 - keep.value   # same slot
+    assert lib.nt_tune_get(1, 1, 1024, ctypes.byref(v), None) == 0 and v.value == p2.value       # power of two
     assert lib.nt_tune_set(1, 1, 700, keep.value) == 0
     assert lib.nt_tune_set(2, 0, 64, 0) == _lib.NT_EINVAL
     assert lib.nt_tune_set(0, 5, 64, 0) == _lib.NT_EINVAL
