@@ -263,6 +263,19 @@ def serving_case(count=1000, rates=(500.0, 2000.0), seed=2):
     return out
 
 
+def served_summary(serving):
+    """p50/p99 of the served request stream at each stated arrival rate, continuous
+    batching beside one-at-a-time plain FP64 (None when the case did not run)."""
+    if not serving:
+        return None
+    out = {"requests": serving["requests"], "arrivals": "Poisson, numpy default_rng(2)"}
+    for key, case in serving.items():
+        if key.startswith("rate_"):
+            out[key] = {m: {k: case[m][k] for k in ("p50_ms", "p99_ms", "throughput_rps")}
+                        for m in ("continuous", "sequential")}
+    return out
+
+
 EXTRA_CASES = ("axes", "refine", "adi", "fp32_storage", "batch", "serving")
 
 
@@ -339,7 +352,7 @@ def main():
 
     pts = N ** 3
     rec = {
-        "metric": "grid pts/s, 512^3 compact d/dx (FP64)",
+        "metric": "Grid pts/s & % HBM peak, 512^3 compact d/dx; p50/p99 served vs plain FP64",
         "value": pts / head["mean_s"],
         "unit": "grid pts/s",
         "n_gpus": 1,
@@ -359,6 +372,8 @@ def main():
                      "traffic_source": "profiles/compact_ddx_512_fp64.md (rocprofv3 --pmc)",
                      "alg_bytes_per_launch": head["alg_bytes"],
                      "kernel_mean_s": head["mean_s"], "kernel_median_s": head["median_s"]},
+        "hbm_peak_frac": head["hbm_frac"],
+        "served": served_summary(extras.get("serving")),
         "cpu_baseline": cpu,
         "wall_s_timed_region": head["wall_s"],
         "extras": extras,
