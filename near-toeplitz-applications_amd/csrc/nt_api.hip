@@ -5,6 +5,8 @@
 // (compact/rhs.py:16-18 derives the extents from the launch grid).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -37,15 +39,19 @@ namespace {
 // launch-variant table: [layout][precision][ceil(log2 n)][n is a power of two].
 // Powers of two take the kernels' compile-time level structure, other lengths
 // the general path, and the two prefer different shapes.  The defaults are the
-// shapes measured fastest on MI355X (compact derivative, about 2^25 points,
-// the other two extents multiples of 64, n = 2^k and n = 3 * 2^(k-2)).
+// shapes measured fastest on MI355X: compact derivatives on aligned grids at
+// n = 2^k and n = 3 * 2^(k-2) (the other two extents multiples of 64), with
+// the 512- and 1024-point strided slots taken from whole 512^3 / 1024^3 fields
+// and the 2048-point strided slot from the ADI y half step (whose lines do not
+// fit the whole-tile kernel).
 int g_tune[2][2][13][2] = {
     // contiguous lines: 0 wave-private transpose, 1 direct chunk loads, 2 workgroup tile
     {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 1}, {1, 1}},
      {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 1}, {0, 1}}},
-    // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512)
-    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 0}, {0, 0}, {1, 0}, {1, 1}, {1, 1}},
-     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 0}, {1, 1}, {0, 0}, {1, 1}, {1, 1}}},
+    // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
+    // 2 persistent whole-tile kernel
+    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 1}, {0, 0}, {1, 2}, {2, 2}, {2, 1}, {2, 1}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 2}, {1, 2}, {2, 0}, {1, 1}, {1, 1}}},
 };
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -56,6 +62,25 @@ int log2_bucket(int n) {
   return b;
 }
 }  // namespace
+
+int64_t nt::persistent_grid(const void* kernel, int threads) {
+  // compute units per device, cached (set once per device; a race only repeats the query)
+  static std::atomic<int> cus_cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
+  int cus = (dev < 64) ? cus_cache[dev].load(std::memory_order_relaxed) : 0;
+  if (cus <= 0) {
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    if (dev < 64) cus_cache[dev].store(cus, std::memory_order_relaxed);
+  }
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) {
+    (void)hipGetLastError();
+    per = 1;
+  }
+  return static_cast<int64_t>(cus) * per;
+}
 
 int nt::tune_get(int layout, int prec, int n) {
   return g_tune[layout & 1][prec & 1][log2_bucket(n)][is_pow2(n) ? 1 : 0];

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "nt_line_kernel.h"
+#include "nt_tile_kernel.h"
 
 namespace nt {
 
@@ -24,8 +25,10 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //   STRIDED 0: tile of 16 adjacent lines, 64 lanes per line (1024 threads)
 //           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads);
 //              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads)
+//           2: persistent whole-tile kernel, 16 lines (8 above 1024 points, and
+//              above 512 for ADI), next tile's loads in flight during this one
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
-constexpr int kNumVariants[2] = {3, 2};
+constexpr int kNumVariants[2] = {3, 3};
 int tune_get(int layout, int prec, int n);
 void tune_set(int layout, int prec, int n, int variant);
 
@@ -87,6 +90,32 @@ inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) ==
                (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
   } while (0)
 
+// Persistent whole-tile kernel for strided lines (nt_tile_kernel.h): a grid of
+// at most as many workgroups as fit on the device at once.
+int64_t persistent_grid(const void* kernel, int threads);
+
+#define NT_GOT(R_, WAVES_)                                                                 \
+  do {                                                                                     \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
+    a.tiles = (nsys_ + (WAVES_)-1) / (WAVES_);                                             \
+    a.total = outer * a.tiles;                                                             \
+    if (a.total <= 0) return hipSuccess;                                                   \
+    const bool full_ = is_full(n, 64, R_);                                                 \
+    const void* k_ = full_                                                                 \
+        ? reinterpret_cast<const void*>(&line_kernel_tile<TIn, TOut, R_, WAVES_, PROD, PREC, true>)  \
+        : reinterpret_cast<const void*>(&line_kernel_tile<TIn, TOut, R_, WAVES_, PROD, PREC, false>); \
+    const int64_t cap_ = persistent_grid(k_, (WAVES_)*64);                                 \
+    const int64_t grid_ = a.total < cap_ ? a.total : cap_;                                 \
+    if (full_)                                                                             \
+      hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, PROD, PREC, true>),      \
+                         dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
+    else                                                                                   \
+      hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, PROD, PREC, false>),     \
+                         dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
+    return hipGetLastError();                                                              \
+  } while (0)
+
 // 16-byte chunk accesses need 16-byte aligned line starts
 inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
   const uintptr_t i = reinterpret_cast<uintptr_t>(a.in), o = reinterpret_cast<uintptr_t>(a.out);
@@ -124,6 +153,24 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     if (n <= 1024) NT_GO(16, 64, 4, 1024);
     if (n <= 2048) NT_GO(32, 64, 4, 2048);
   } else {
+    if (variant == 2) {
+      // persistent whole-tile kernel, next tile prefetched into registers
+      if constexpr (PROD != P_ADI) {
+        if (n <= 64) NT_GOT(1, 16);
+        if (n <= 128) NT_GOT(2, 16);
+        if (n <= 256) NT_GOT(4, 16);
+        if (n <= 512) NT_GOT(8, 16);
+        if (n <= 1024) NT_GOT(16, 16);
+        if (n <= 2048) NT_GOT(32, 8);
+      } else {
+        if (n <= 64) NT_GOT(1, 16);
+        if (n <= 128) NT_GOT(2, 16);
+        if (n <= 256) NT_GOT(4, 16);
+        if (n <= 512) NT_GOT(8, 16);
+        if (n <= 1024) NT_GOT(16, 8);
+        // 2048-point ADI lines do not fit a whole tile in LDS: blocked kernel below
+      }
+    }
     if (variant == 1) {
       // 32 adjacent lines per tile (256-byte FP64 rows), 32 lanes per line
       if (n <= 32) NT_GOS(1, 32, 16, 32);

@@ -38,6 +38,7 @@ struct LineArgs {
   int64_t ostride;     // distance between outer slabs
   int64_t phys;        // lines per slab including halo lines (ADI boundary lines)
   int64_t tiles;       // tiles per slab
+  int64_t total;       // tiles over all slabs (persistent kernels)
   double s1, s2;       // producer scalars: compact 3/(4h), 1/(2h); ADI r/2, 1-r
   int32_t sweeps;      // refinement sweeps (PREC_F32R)
   int32_t pad_;
@@ -547,6 +548,12 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
 // chunk.  Lines are solved by segments of W lanes, so the chunk is R = n/W
 // rows (small for W = 64 or 32).  One workgroup barrier separates loading and
 // copying each block; blocks alternate between two LDS buffers.
+// Blocks of a strided tile kept in flight (see the pipeline in the kernel):
+// deeper for small per-thread blocks, where the registers are cheap.
+#ifndef NT_STRIDED_DEPTH
+#define NT_STRIDED_DEPTH(PF) (((PF) <= 2) ? 4 : (((PF) <= 4) ? 2 : 1))
+#endif
+
 template <typename TIn, typename TOut, int R, int W, int WAVES, int E, int PROD, int PREC, bool FULL>
 __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs args_by_value) {
   (void)args_by_value;
@@ -600,16 +607,20 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs
   for (int k = 0; k < R; ++k) f[k] = 0.0;
   double bnd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 
-  // Block loads are software-pipelined: the values of block b+1 are fetched
-  // into registers (pf) before the barrier and copy of block b, so their
-  // latency overlaps the copy.  PF: values per thread per block.
+  // Block loads are software-pipelined DEPTH blocks deep: the values of block
+  // b+DEPTH are fetched into registers as soon as block b has been moved to
+  // LDS, so up to DEPTH blocks are in flight while earlier ones are copied.
+  // PF: values per thread per block; the register ring holds DEPTH blocks.
   constexpr int VPR = ROWS / (16 / static_cast<int>(sizeof(TIn)));   // vectors per block row
   constexpr int PFV = (VPR * E + NT - 1) / NT;                        // vector loads per thread
   constexpr int PFS = (ROWS * E + NT - 1) / NT;                       // scalar loads per thread
   constexpr int VIN_ = 16 / static_cast<int>(sizeof(TIn));
   constexpr int PF = (PFV * VIN_ > PFS) ? PFV * VIN_ : PFS;
-  double pf[PF];
-  auto fetch = [&](int b) {
+  constexpr int NBMAX = (W * R + E - 1) / E;
+  constexpr int DEPTH_ = NT_STRIDED_DEPTH(PF);
+  constexpr int DEPTH = (DEPTH_ < NBMAX) ? DEPTH_ : NBMAX;
+  double ring[DEPTH][PF];
+  auto fetch = [&](int b, double (&pf)[PF]) {
     const int e0 = b * E;
     const int ecount = (n - e0 < E) ? n - e0 : E;
     if (vec_in) {
@@ -642,7 +653,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs
       }
     }
   };
-  auto deposit = [&](int b) {
+  auto deposit = [&](int b, const double (&pf)[PF]) {
     const int e0 = b * E;
     const int buf = b & 1;
     const int ecount = (n - e0 < E) ? n - e0 : E;
@@ -666,13 +677,9 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs
     }
   };
 
-  fetch(0);
-  for (int b = 0; b < nblk; ++b) {
+  auto consume = [&](int b) {
     const int e0 = b * E;
     const int buf = b & 1;
-    deposit(b);
-    if (b + 1 < nblk) fetch(b + 1);
-    __syncthreads();
     if (my_blk == b && active) {
       const int el0 = my_e0 - e0;
 #pragma unroll
@@ -697,8 +704,24 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs
         if (e >= e0 && e < e0 + E) bnd[i] = lds[at(buf, ql + HALO, e - e0)];
       }
     }
-    // block b+1 is written to the other buffer; this buffer is rewritten (block
-    // b+2) only after the next barrier, which every reader of block b has passed
+  };
+
+#pragma unroll
+  for (int j = 0; j < DEPTH; ++j)
+    if (j < nblk) fetch(j, ring[j]);
+  for (int b0 = 0; b0 < nblk; b0 += DEPTH) {
+#pragma unroll
+    for (int j = 0; j < DEPTH; ++j) {
+      const int b = b0 + j;
+      if (b < nblk) {                            // uniform across the workgroup
+        deposit(b, ring[j]);
+        if (b + DEPTH < nblk) fetch(b + DEPTH, ring[j]);
+        __syncthreads();
+        consume(b);
+        // block b+1 goes to the other LDS buffer; this one is rewritten (block
+        // b+2) only after the next barrier, which every reader of block b has passed
+      }
+    }
   }
 
   double d[R];

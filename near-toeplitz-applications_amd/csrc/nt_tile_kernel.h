@@ -1,0 +1,242 @@
+// Persistent whole-tile kernel for strided lines (y- and z-lines of a field).
+//
+// A tile is TS adjacent lines (one wavefront each, 64 lanes x R rows) whose
+// rows are contiguous runs of TS elements in memory.  The whole tile sits in
+// LDS at once, so a tile costs three workgroup barriers, not two per row
+// block.  The workgroup is persistent (the order in which it walks tiles is
+// described at the loop) and fetches the next tile's values into registers
+// while the current tile is solved and stored (for R up to 16; longer lines
+// fetch after the solve, where the registers are free again).
+//
+//   fetch(t0) ; loop { barrier ; deposit(t) ; [fetch(next)] ; barrier ; pull
+//                      chunks ; solve ; barrier (ADI halo) ; write x ;
+//                      barrier ; [fetch(next)] ; store tile }
+//
+// LDS holds ROWS x n doubles (plus chunk padding), which bounds n * ROWS; the
+// launcher offers this kernel only where that fits (see nt_launch.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nt_line_kernel.h"
+
+// Largest rows-per-lane R whose next-tile prefetch is issued before the solve.
+#ifndef NT_TILE_EARLY_MAX_R
+#define NT_TILE_EARLY_MAX_R 16
+#endif
+
+namespace nt {
+
+template <typename TIn, typename TOut, int R, int WAVES, int PROD, int PREC, bool FULL>
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
+  constexpr int W = 64;
+  constexpr int NT = WAVES * 64;
+  constexpr int TS = WAVES;                      // lines per tile, one per wavefront
+  constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
+  constexpr int ROWS = TS + 2 * HALO;            // staged lines
+  constexpr int NMAX = W * R;
+  constexpr int CP = (R >= 4) ? 2 : 0;
+  constexpr int EP = (NMAX / R) * (R + CP) + 2;  // LDS doubles per staged line
+  static_assert(ROWS * EP * 8 <= 160 * 1024, "tile does not fit in LDS");
+  __shared__ __attribute__((aligned(16))) double lds[ROWS * EP];
+
+  // per-thread share of a tile: vector path (16-byte row pieces) or scalar path
+  constexpr int VIN = 16 / static_cast<int>(sizeof(TIn));
+  constexpr int VPR = (ROWS % VIN == 0) ? ROWS / VIN : 1;        // vectors per row
+  constexpr int PFV = (VPR * NMAX + NT - 1) / NT;
+  constexpr int PFS = (ROWS * NMAX + NT - 1) / NT;
+  constexpr int PF = (PFV * VIN > PFS) ? PFV * VIN : PFS;
+  constexpr int VOUT = 16 / static_cast<int>(sizeof(TOut));
+  constexpr int VPO = TS / VOUT;                                  // output vectors per row
+
+  const NtPlanK& P = args.plan;
+  const int n = FULL ? NMAX : P.n;
+  const int tid = threadIdx.x;
+  const int ql = tid >> 6;                       // my line in the tile
+  const int lane = tid & 63;
+  const int my_e0 = lane * R;
+  const int64_t nsys = args.phys - 2 * HALO;
+  const int64_t total = args.total;
+  auto at = [&](int row, int el) -> int { return row * EP + el + (el / R) * CP; };
+
+  struct Tile {
+    int64_t p0, base;
+    int rows_valid;
+    bool vec_in, vec_out;
+  };
+  auto geometry = [&](int64_t t) -> Tile {
+    Tile g;
+    const int64_t o = t / args.tiles;
+    g.p0 = (t - o * args.tiles) * TS;
+    g.base = o * args.ostride + g.p0;            // pstride == 1 for strided lines
+    const int64_t rv = args.phys - g.p0;
+    g.rows_valid = rv < ROWS ? static_cast<int>(rv) : ROWS;
+    const TIn* in = static_cast<const TIn*>(args.in) + g.base;
+    TOut* out = static_cast<TOut*>(args.out) + g.base;
+    g.vec_in = (HALO == 0) && (ROWS % VIN == 0) && (g.rows_valid == ROWS) &&
+               (reinterpret_cast<uintptr_t>(in) % 16 == 0) && ((args.estride * sizeof(TIn)) % 16 == 0);
+    g.vec_out = (HALO == 0) && (TS % VOUT == 0) && (g.p0 + TS <= nsys) &&
+                (reinterpret_cast<uintptr_t>(out) % 16 == 0) && ((args.estride * sizeof(TOut)) % 16 == 0);
+    return g;
+  };
+
+  double pf[PF];
+  auto fetch = [&](const Tile& g) {
+    const TIn* __restrict__ in = static_cast<const TIn*>(args.in) + g.base;
+    if (g.vec_in) {
+#pragma unroll
+      for (int i = 0; i < PFV; ++i) {
+        const int idx = tid + i * NT;
+        const int vr = idx % VPR, el = idx / VPR;
+        if (el < n) {
+          const TIn* src = in + static_cast<int64_t>(el) * args.estride + vr * VIN;
+          if constexpr (sizeof(TIn) == 8) {
+            const nt_dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(src));
+            pf[i * 2] = v.x;
+            pf[i * 2 + 1] = v.y;
+          } else {
+            const nt_flt4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_flt4*>(src));
+            pf[i * 4] = v.x;
+            pf[i * 4 + 1] = v.y;
+            pf[i * 4 + 2] = v.z;
+            pf[i * 4 + 3] = v.w;
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * NT;
+        const int r = idx % ROWS, el = idx / ROWS;
+        if (el < n && r < g.rows_valid)
+          pf[i] = static_cast<double>(in[r + static_cast<int64_t>(el) * args.estride]);
+      }
+    }
+  };
+  auto deposit = [&](const Tile& g) {
+    if (g.vec_in) {
+#pragma unroll
+      for (int i = 0; i < PFV; ++i) {
+        const int idx = tid + i * NT;
+        const int vr = idx % VPR, el = idx / VPR;
+        if (el < n) {
+#pragma unroll
+          for (int c = 0; c < VIN; ++c) lds[at(vr * VIN + c, el)] = pf[i * VIN + c];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * NT;
+        const int r = idx % ROWS, el = idx / ROWS;
+        if (el < n && r < g.rows_valid) lds[at(r, el)] = pf[i];
+      }
+    }
+  };
+
+  constexpr bool EARLY = (R <= NT_TILE_EARLY_MAX_R);   // prefetch before the solve
+  // Tiles of this workgroup (the launcher keeps gridDim.x <= total, so none is
+  // empty).  Lines spread over several slabs (y-lines: one slab per z-plane):
+  // a balanced contiguous range, so the workgroups running together work on
+  // different slabs.  One slab (z-lines, rows a whole plane apart): every
+  // gridDim.x-th tile, so the workgroups running together read neighbouring
+  // pieces of the same rows.  Measured on MI355X, each order is the faster one
+  // for its case.
+  const bool ranges = args.ostride != 0;
+  const int64_t t_begin = ranges ? (static_cast<int64_t>(blockIdx.x) * total) / gridDim.x
+                                 : static_cast<int64_t>(blockIdx.x);
+  const int64_t t_end = ranges ? ((static_cast<int64_t>(blockIdx.x) + 1) * total) / gridDim.x : total;
+  const int64_t t_step = ranges ? 1 : static_cast<int64_t>(gridDim.x);
+  int64_t t = t_begin;
+  Tile g = geometry(t);
+  fetch(g);
+  for (;;) {
+    __syncthreads();                             // the previous tile's stores have read the LDS
+    deposit(g);
+    const int64_t tn = t + t_step;
+    const bool more = tn < t_end;                // uniform
+    Tile gn = g;
+    if (more) gn = geometry(tn);
+    if (EARLY && more) fetch(gn);                // the next tile's loads fly during this one
+    __syncthreads();
+
+    const bool active = (g.p0 + ql) < nsys;      // uniform across the wavefront
+    double x[R];
+    if (active) {
+      double f[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int e = my_e0 + k;
+        double v = 0.0;
+        if (FULL || e < n) {
+          const double c = lds[at(ql + HALO, e)];
+          if (PROD == P_ADI) {
+            v = (e == 0 || e == n - 1) ? c : args.s1 * (lds[at(ql, e)] + lds[at(ql + 2, e)]) + args.s2 * c;
+          } else {
+            v = c;
+          }
+        }
+        f[k] = v;
+      }
+      double d[R];
+      if (PROD == P_COMPACT) {
+        double bnd[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bnd[i] = lds[at(ql, (i < 3) ? i : n + 2 - i)];
+        const double fl = shup<W>(f[R - 1], 1);
+        const double fr = shdn<W>(f[0], 1);
+        compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) d[k] = f[k];
+      }
+      solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
+    }
+    // the solution overwrites the rows just read: the wavefront's own line, and
+    // with a halo also the neighbours' lines, which other wavefronts read
+    if (HALO) __syncthreads();
+    else wave_lds_sync();
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (FULL || my_e0 + k < n) lds[at(ql + HALO, my_e0 + k)] = x[k];
+    }
+    __syncthreads();
+    if (!EARLY && more) fetch(gn);               // in flight during this tile's stores
+
+    TOut* __restrict__ out = static_cast<TOut*>(args.out) + g.base;
+    if (g.vec_out) {
+      for (int idx = tid; idx < VPO * n; idx += NT) {
+        const int vr = idx % VPO, el = idx / VPO;
+        TOut* dst = out + static_cast<int64_t>(el) * args.estride + vr * VOUT;
+        if constexpr (sizeof(TOut) == 8) {
+          nt_dbl2 v;
+          v.x = lds[at(vr * 2, el)];
+          v.y = lds[at(vr * 2 + 1, el)];
+          __builtin_nontemporal_store(v, reinterpret_cast<nt_dbl2*>(dst));
+        } else {
+          nt_flt4 v;
+          v.x = static_cast<float>(lds[at(vr * 4, el)]);
+          v.y = static_cast<float>(lds[at(vr * 4 + 1, el)]);
+          v.z = static_cast<float>(lds[at(vr * 4 + 2, el)]);
+          v.w = static_cast<float>(lds[at(vr * 4 + 3, el)]);
+          __builtin_nontemporal_store(v, reinterpret_cast<nt_flt4*>(dst));
+        }
+      }
+    } else {
+      for (int idx = tid; idx < TS * n; idx += NT) {
+        const int r = idx % TS, el = idx / TS;
+        if (g.p0 + r < nsys)
+          out[(r + HALO) + static_cast<int64_t>(el) * args.estride] = static_cast<TOut>(lds[at(r + HALO, el)]);
+      }
+    }
+    if (!more) break;
+    t = tn;
+    g = gn;
+  }
+}
+
+}  // namespace nt

@@ -56,7 +56,7 @@ def _time(fn, reps):
     return a.elapsed_time(b) / reps * 1e-3
 
 
-def tune(ns=(64, 128, 256, 512, 1024, 2048), points=1 << 25, reps=10, precisions=("fp64", "fp32_refine"),
+def tune(ns=(64, 128, 256, 512, 1024, 2048), points=1 << 27, reps=10, precisions=("fp64", "fp32_refine"),
          verbose=False):
     """Time every variant for contiguous (layout 0) and strided (layout 1) lines
     at each precision mode, on fields of about ``points`` points, and apply the

@@ -137,15 +137,17 @@ def test_tuning_table_set_get_and_validation():
                 assert lib.nt_tune_get(layout, prec, n, ctypes.byref(v), ctypes.byref(nv)) == 0
                 assert 0 <= v.value < nv.value
     keep = ctypes.c_int()
-    assert lib.nt_tune_get(1, 1, 700, ctypes.byref(keep), None) == 0
+    assert lib.nt_tune_get(1, 1, 700, ctypes.byref(keep), ctypes.byref(nv)) == 0
+    other = (keep.value + 1) % nv.value
     p2 = ctypes.c_int()
     assert lib.nt_tune_get(1, 1, 1024, ctypes.byref(p2), None) == 0
-    assert lib.nt_tune_set(1, 1, 700, 1 - keep.value) == 0
-    assert lib.nt_tune_get(1, 1, 900, ctypes.byref(v), None) == 0 and v.value == 1 - keep.value   # same slot
+    assert lib.nt_tune_set(1, 1, 700, other) == 0
+    assert lib.nt_tune_get(1, 1, 900, ctypes.byref(v), None) == 0 and v.value == other         # same slot
     assert lib.nt_tune_get(1, 1, 1024, ctypes.byref(v), None) == 0 and v.value == p2.value       # power of two
     assert lib.nt_tune_set(1, 1, 700, keep.value) == 0
     assert lib.nt_tune_set(2, 0, 64, 0) == _lib.NT_EINVAL
     assert lib.nt_tune_set(0, 5, 64, 0) == _lib.NT_EINVAL
     assert lib.nt_tune_set(0, 0, 1, 0) == _lib.NT_EINVAL
     assert lib.nt_tune_set(0, 0, 64, 3) == _lib.NT_EINVAL
+    assert lib.nt_tune_set(1, 0, 64, 3) == _lib.NT_EINVAL
     assert b"variant" in lib.nt_last_error()
