@@ -50,8 +50,8 @@ int g_tune[2][2][13][2] = {
      {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 1}, {0, 1}}},
     // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
     // 2 persistent whole-tile kernel
-    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 1}, {0, 0}, {1, 2}, {2, 2}, {2, 1}, {2, 1}},
-     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 2}, {1, 2}, {2, 0}, {1, 1}, {1, 1}}},
+    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 1}, {0, 0}, {2, 2}, {2, 2}, {2, 1}, {2, 1}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 2}, {2, 2}, {2, 2}, {1, 1}, {1, 1}}},
 };
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }

@@ -127,18 +127,20 @@ __device__ __forceinline__ void solve_line(double* row, int n, int prod, int pre
     if (FULL || my_e0 + k < n) row[my_e0 + k] = x[k];
 }
 
-__global__ void __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
-batch_kernel(const BatchArgs args) {
+__global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
   __shared__ __attribute__((aligned(16))) double lds[BATCH_TS * BATCH_EP];
   __shared__ __attribute__((aligned(16))) NtPlanK s_plan;   // level table of the current job
   __shared__ int64_t s_tile;
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-  const int lane = tid & 63;
+  // reset through an opaque move at every tile, so the per-thread offsets
+  // derived from it are recomputed per tile, not hoisted and held across the loop
+  int tid = threadIdx.x;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   int j = 0;                        // current job: tiles come in increasing order
   int plan_in_lds = -1;
 
   for (;;) {
+    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (BATCH_NT - 1);   // the mask keeps its range known
     if (tid == 0) s_tile = static_cast<int64_t>(atomicAdd(args.counter, 1u));
     __syncthreads();
     // the tile index is the same for the whole workgroup: keep it (and the

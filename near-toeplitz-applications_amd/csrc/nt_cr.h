@@ -136,6 +136,14 @@ __device__ __forceinline__ int opaque_zero() {
 }
 __device__ __forceinline__ void level_fence() { __builtin_amdgcn_sched_barrier(0); }
 
+// A per-lane zero the compiler cannot see through (keeps values derived from
+// it from being hoisted out of a loop).
+__device__ __forceinline__ int opaque_vzero() {
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
 // Inside a branch on a wave-uniform condition: keeps the branch (an empty
 // volatile asm cannot be speculated), so a patch that applies to one register
 // position is not computed and selected at every position.

@@ -63,9 +63,11 @@ __device__ __forceinline__ void st_from_double(T* p, double v) {
   *p = static_cast<T>(v);
 }
 
-// residual r = d - A x for the level-0 near-Toeplitz matrix, chunked layout
-template <int R, int W>
-__device__ __forceinline__ void residual(const double (&d)[R], const double (&x)[R], double (&r)[R],
+// residual r = d - A x for the level-0 near-Toeplitz matrix, chunked layout,
+// formed in FP64 and rounded to the type of r (the FP32 correction solve takes
+// it as float, so no FP64 copy of r is kept)
+template <int R, int W, typename TR>
+__device__ __forceinline__ void residual(const double (&d)[R], const double (&x)[R], TR (&r)[R],
                                          const NtPlanK& P, int lane) {
   const double xl = shup<W>(x[R - 1], 1);
   const double xr = shdn<W>(x[0], 1);
@@ -83,7 +85,7 @@ __device__ __forceinline__ void residual(const double (&d)[R], const double (&x)
     } else {
       ax = P.ai * left + P.bi * x[k] + P.ci * right;
     }
-    r[k] = (e < n) ? d[k] - ax : 0.0;
+    r[k] = static_cast<TR>((e < n) ? d[k] - ax : 0.0);
   }
 }
 
@@ -132,10 +134,7 @@ __device__ __forceinline__ void solve_chunk(const double (&d)[R], double (&x)[R]
 #pragma unroll
     for (int k = 0; k < R; ++k) x[k] = static_cast<double>(s[k]);
     for (int it = 0; it < sweeps; ++it) {
-      double r[R];
-      residual<R, W>(d, x, r, P, lane);
-#pragma unroll
-      for (int k = 0; k < R; ++k) s[k] = static_cast<float>(r[k]);
+      residual<R, W>(d, x, s, P, lane);
       cr_solve_regs<float, R, W, FULL>(s, P.f, P.nlev, lane);
 #pragma unroll
       for (int k = 0; k < R; ++k) x[k] += static_cast<double>(s[k]);
@@ -571,7 +570,9 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs
 
   const NtPlanK& P = args.plan;
   const int n = FULL ? W * R : P.n;
-  const int tid = threadIdx.x;
+  // refreshed through an opaque move in every block of the load loop, so the
+  // per-thread offsets derived from it are recomputed, not held across the loop
+  int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int seg = (tid & 63) / W;
   const int lane = tid & (W - 1);
@@ -714,6 +715,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_strided(const LineArgs
     for (int j = 0; j < DEPTH; ++j) {
       const int b = b0 + j;
       if (b < nblk) {                            // uniform across the workgroup
+        tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);   // the mask keeps its range known
         deposit(b, ring[j]);
         if (b + DEPTH < nblk) fetch(b + DEPTH, ring[j]);
         __syncthreads();

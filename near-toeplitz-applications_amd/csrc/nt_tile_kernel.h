@@ -54,9 +54,12 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
 
   const NtPlanK& P = args.plan;
   const int n = FULL ? NMAX : P.n;
-  const int tid = threadIdx.x;
-  const int ql = tid >> 6;                       // my line in the tile
-  const int lane = tid & 63;
+  // Reset at the top of every tile through an opaque move, so the per-thread
+  // load, store and LDS offsets derived from it are recomputed for each tile
+  // instead of being hoisted out of the tile loop and held (spilled) across it.
+  int tid = threadIdx.x;
+  const int ql = threadIdx.x >> 6;               // my line in the tile
+  const int lane = threadIdx.x & 63;
   const int my_e0 = lane * R;
   const int64_t nsys = args.phys - 2 * HALO;
   const int64_t total = args.total;
@@ -154,6 +157,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
   Tile g = geometry(t);
   fetch(g);
   for (;;) {
+    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);   // the mask keeps its range known
     __syncthreads();                             // the previous tile's stores have read the LDS
     deposit(g);
     const int64_t tn = t + t_step;
