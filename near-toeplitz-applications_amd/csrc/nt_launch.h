@@ -90,6 +90,12 @@ inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) ==
                (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
   } while (0)
 
+// wavefronts per workgroup of the wave-private-transpose kernel for 257..512-point
+// lines (one: measured 1-3% faster than four on MI355X at 512^3)
+#ifndef NT_WCT_WAVES_512
+#define NT_WCT_WAVES_512 1
+#endif
+
 // Persistent whole-tile kernel for strided lines (nt_tile_kernel.h): a grid of
 // at most as many workgroups as fit on the device at once.
 int64_t persistent_grid(const void* kernel, int threads);
@@ -133,7 +139,7 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     if (variant == 0 && aligned && a.pstride == n && n % vin == 0 && n % vout == 0) {
       if (n <= 128) NT_GOW(2, 64, 4);
       if (n <= 256) NT_GOW(4, 64, 4);
-      if (n <= 512) NT_GOW(8, 64, 4);
+      if (n <= 512) NT_GOW(8, 64, NT_WCT_WAVES_512);
       if (n <= 1024) NT_GOW(16, 64, 4);
       if (n <= 2048) NT_GOW(32, 64, 2);
     }
