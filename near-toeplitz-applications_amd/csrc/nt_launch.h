@@ -26,7 +26,8 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads);
 //              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads)
 //           2: persistent whole-tile kernel, 16 lines (8 above 1024 points, and
-//              above 512 for ADI), next tile's loads in flight during this one
+//              above 512 for ADI), next tile's loads in flight during this one;
+//              FP32-stored compact derivatives: 32 lines, two per wavefront
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 constexpr int kNumVariants[2] = {3, 3};
 int tune_get(int layout, int prec, int n);
@@ -122,6 +123,27 @@ int64_t persistent_grid(const void* kernel, int threads);
     return hipGetLastError();                                                              \
   } while (0)
 
+// FP32-stored compact derivative along strided lines: 32-line tiles (nt_tile_kernel.h)
+#define NT_GOTF(R_)                                                                        \
+  do {                                                                                     \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    a.tiles = (a.phys + 31) / 32;                                                          \
+    a.total = outer * a.tiles;                                                             \
+    if (a.total <= 0) return hipSuccess;                                                   \
+    const bool full_ = is_full(n, 64, R_);                                                 \
+    const void* k_ = full_ ? reinterpret_cast<const void*>(&line_kernel_tile_f32<R_, PREC, true>)   \
+                           : reinterpret_cast<const void*>(&line_kernel_tile_f32<R_, PREC, false>); \
+    const int64_t cap_ = persistent_grid(k_, 1024);                                        \
+    const int64_t grid_ = a.total < cap_ ? a.total : cap_;                                 \
+    if (full_)                                                                             \
+      hipLaunchKernelGGL((line_kernel_tile_f32<R_, PREC, true>), dim3(static_cast<unsigned>(grid_)), \
+                         dim3(1024), 0, st, a);                                            \
+    else                                                                                   \
+      hipLaunchKernelGGL((line_kernel_tile_f32<R_, PREC, false>), dim3(static_cast<unsigned>(grid_)), \
+                         dim3(1024), 0, st, a);                                            \
+    return hipGetLastError();                                                              \
+  } while (0)
+
 // 16-byte chunk accesses need 16-byte aligned line starts
 inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
   const uintptr_t i = reinterpret_cast<uintptr_t>(a.in), o = reinterpret_cast<uintptr_t>(a.out);
@@ -161,7 +183,15 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
   } else {
     if (variant == 2) {
       // persistent whole-tile kernel, next tile prefetched into registers
-      if constexpr (PROD != P_ADI) {
+      if constexpr (std::is_same_v<TIn, float> && std::is_same_v<TOut, double> && PROD == P_COMPACT) {
+        // FP32 rows: 32-line tiles, two lines per wavefront (lines up to 1024 points)
+        if (n <= 64) NT_GOTF(1);
+        if (n <= 128) NT_GOTF(2);
+        if (n <= 256) NT_GOTF(4);
+        if (n <= 512) NT_GOTF(8);
+        if (n <= 1024) NT_GOTF(16);
+        if (n <= 2048) NT_GOT(32, 8);
+      } else if constexpr (PROD != P_ADI) {
         if (n <= 64) NT_GOT(1, 16);
         if (n <= 128) NT_GOT(2, 16);
         if (n <= 256) NT_GOT(4, 16);

@@ -243,4 +243,163 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
   }
 }
 
+
+// FP32-stored fields (compact derivative, FP64 result): a 16-line tile of FP32
+// rows reads 64-byte runs, half of what a row of FP64 gives.  This variant
+// takes 32-line tiles (128-byte FP32 runs), two lines per wavefront: the tile
+// sits in LDS as float, each wavefront pulls both of its lines' chunks into
+// registers, and the FP64 results of each half (16 lines, 128-byte FP64 runs)
+// are staged through the same LDS in turn.  No next-tile prefetch: the
+// registers hold the second line while the first is solved.
+template <int R, int PREC, bool FULL>
+__global__ void __launch_bounds__(1024) line_kernel_tile_f32(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
+  constexpr int W = 64;
+  constexpr int NT = 1024;
+  constexpr int HALF = 16;                       // lines solved at once, one per wavefront
+  constexpr int TS = 2 * HALF;                   // lines per tile
+  constexpr int NMAX = W * R;
+  constexpr int EPF = (NMAX / R) * (R + 1) + 1;  // floats per staged input line (odd chunk stride)
+  constexpr int CP = (R >= 4) ? 2 : 0;
+  constexpr int EPD = (NMAX / R) * (R + CP) + 2; // doubles per staged output line
+  constexpr int IN_BYTES = TS * EPF * 4;
+  constexpr int OUT_BYTES = HALF * EPD * 8;
+  constexpr int LDS_BYTES = IN_BYTES > OUT_BYTES ? IN_BYTES : OUT_BYTES;
+  static_assert(LDS_BYTES <= 160 * 1024, "tile does not fit in LDS");
+  __shared__ __attribute__((aligned(16))) double smem[LDS_BYTES / 8];
+  float* ldf = reinterpret_cast<float*>(smem);
+  double* ldd = smem;
+  constexpr int VPR = TS / 4;                    // 16-byte vectors per input row
+  constexpr int PFV = (VPR * NMAX + NT - 1) / NT;
+  constexpr int PFS = (TS * NMAX + NT - 1) / NT;
+  constexpr int VPO = HALF / 2;                  // 16-byte vectors per output row
+
+  const NtPlanK& P = args.plan;
+  const int n = FULL ? NMAX : P.n;
+  int tid = threadIdx.x;                         // refreshed per tile (see line_kernel_tile)
+  const int ql = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int my_e0 = lane * R;
+  const int64_t nsys = args.phys;
+  const int64_t total = args.total;
+  auto atf = [&](int row, int el) -> int { return row * EPF + el + el / R; };
+  auto atd = [&](int row, int el) -> int { return row * EPD + el + (el / R) * CP; };
+
+  const bool ranges = args.ostride != 0;         // tile order as in line_kernel_tile
+  const int64_t t_begin = ranges ? (static_cast<int64_t>(blockIdx.x) * total) / gridDim.x
+                                 : static_cast<int64_t>(blockIdx.x);
+  const int64_t t_end = ranges ? ((static_cast<int64_t>(blockIdx.x) + 1) * total) / gridDim.x : total;
+  const int64_t t_step = ranges ? 1 : static_cast<int64_t>(gridDim.x);
+
+  for (int64_t t = t_begin; t < t_end; t += t_step) {
+    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);
+    const int64_t o = t / args.tiles;
+    const int64_t p0 = (t - o * args.tiles) * TS;
+    const int64_t base = o * args.ostride + p0;
+    const int64_t rv = nsys - p0;
+    const int rows_valid = rv < TS ? static_cast<int>(rv) : TS;
+    const float* __restrict__ in = static_cast<const float*>(args.in) + base;
+    double* __restrict__ out = static_cast<double*>(args.out) + base;
+    const bool vec_in = (rows_valid == TS) && (reinterpret_cast<uintptr_t>(in) % 16 == 0) &&
+                        ((args.estride * 4) % 16 == 0);
+
+    __syncthreads();                             // the previous tile's last stores have read the LDS
+    if (vec_in) {
+      nt_flt4 v[PFV];
+#pragma unroll
+      for (int i = 0; i < PFV; ++i) {
+        const int idx = tid + i * NT;
+        const int vr = idx % VPR, el = idx / VPR;
+        if (el < n)
+          v[i] = __builtin_nontemporal_load(
+              reinterpret_cast<const nt_flt4*>(in + static_cast<int64_t>(el) * args.estride + vr * 4));
+      }
+#pragma unroll
+      for (int i = 0; i < PFV; ++i) {
+        const int idx = tid + i * NT;
+        const int vr = idx % VPR, el = idx / VPR;
+        if (el < n) {
+          ldf[atf(vr * 4, el)] = v[i].x;
+          ldf[atf(vr * 4 + 1, el)] = v[i].y;
+          ldf[atf(vr * 4 + 2, el)] = v[i].z;
+          ldf[atf(vr * 4 + 3, el)] = v[i].w;
+        }
+      }
+    } else {
+      float v[PFS];
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * NT;
+        const int r = idx % TS, el = idx / TS;
+        v[i] = (el < n && r < rows_valid) ? in[r + static_cast<int64_t>(el) * args.estride] : 0.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * NT;
+        const int r = idx % TS, el = idx / TS;
+        if (el < n && r < rows_valid) ldf[atf(r, el)] = v[i];
+      }
+    }
+    __syncthreads();
+
+    // both lines' chunks and closure values into registers; the LDS is then free
+    float fa[R], fb[R], ba[6], bb[6];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int e = my_e0 + k;
+      fa[k] = (FULL || e < n) ? ldf[atf(ql, e)] : 0.0f;
+      fb[k] = (FULL || e < n) ? ldf[atf(ql + HALF, e)] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int e = (i < 3) ? i : n + 2 - i;
+      ba[i] = ldf[atf(ql, e)];
+      bb[i] = ldf[atf(ql + HALF, e)];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int line = ql + h * HALF;            // tile-local line of this wavefront
+      const bool active = (p0 + line) < nsys;    // uniform across the wavefront
+      if (active) {
+        double f[R], bnd[6];
+#pragma unroll
+        for (int k = 0; k < R; ++k) f[k] = static_cast<double>(h == 0 ? fa[k] : fb[k]);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bnd[i] = static_cast<double>(h == 0 ? ba[i] : bb[i]);
+        const double fl = shup<W>(f[R - 1], 1);
+        const double fr = shdn<W>(f[0], 1);
+        double d[R], x[R];
+        compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+        solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (FULL || my_e0 + k < n) ldd[atd(ql, my_e0 + k)] = x[k];
+      }
+      __syncthreads();
+      const int64_t q0 = p0 + h * HALF;          // first line of this half
+      double* __restrict__ dst = out + h * HALF;
+      const bool vec_out = (q0 + HALF <= nsys) && (reinterpret_cast<uintptr_t>(dst) % 16 == 0) &&
+                           ((args.estride * 8) % 16 == 0);
+      if (vec_out) {
+        for (int idx = tid; idx < VPO * n; idx += NT) {
+          const int vr = idx % VPO, el = idx / VPO;
+          nt_dbl2 v;
+          v.x = ldd[atd(vr * 2, el)];
+          v.y = ldd[atd(vr * 2 + 1, el)];
+          __builtin_nontemporal_store(v, reinterpret_cast<nt_dbl2*>(dst + static_cast<int64_t>(el) * args.estride + vr * 2));
+        }
+      } else {
+        for (int idx = tid; idx < HALF * n; idx += NT) {
+          const int r = idx % HALF, el = idx / HALF;
+          if (q0 + r < nsys) dst[r + static_cast<int64_t>(el) * args.estride] = ldd[atd(r, el)];
+        }
+      }
+      if (h == 0) __syncthreads();               // the second half rewrites the staging area
+    }
+  }
+}
+
 }  // namespace nt

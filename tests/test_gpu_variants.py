@@ -46,3 +46,31 @@ def test_every_variant_matches_oracle(dev, precision, n, axis):
     scale = np.abs(want).max()
     for v in range(1, nv):
         assert np.abs(outs[v] - outs[0]).max() / scale < 1e-13, v
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("n", [64, 100, 256, 512, 768, 1024, 2048])
+@pytest.mark.parametrize("axis", [1, 0])
+def test_every_strided_variant_fp32_storage(dev, precision, n, axis):
+    """FP32-stored fields along strided lines (the 32-line tile kernel for FP32
+    rows among them; ragged last tile with 40 lines): every variant equals the
+    oracle's derivative of the same FP32 values."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.autotune import _set, current
+    from ntamd.workloads import compact_field
+    shape = (3, n, 40) if axis == 1 else (n, 2, 40)
+    f32 = compact_field(shape, seed=n + 7).astype(np.float32)
+    h = 1.0 / (n - 1)
+    want = compact.derivative_banded(f32.astype(np.float64), h, axis)
+    fd = torch.tensor(f32, device=dev)
+    plan = Plan(n, COMPACT_COEFFS)
+    keep, nv = current(1, n, precision)
+    try:
+        for v in range(nv):
+            _set(1, precision, n, v)
+            out = torch.full(shape, float("nan"), dtype=torch.float64, device=dev)
+            core.compact_derivative(plan, fd, out, h, axis, precision)
+            err = np.abs(out.cpu().numpy() - want).max() / np.abs(want).max()
+            assert err < TOL[precision], (v, err)
+    finally:
+        _set(1, precision, n, keep)
