@@ -129,8 +129,16 @@ def adi_case(N, r, nsteps, reps):
         ts.append(a.elapsed_time(b) * 1e-3)
     t = statistics.median(ts)
     half_bytes = 16 * N * N
+    # the one-GPU partitioned form: 16 subdomains of 128 per line plus the reduced
+    # 32-unknown system, one launch per x-implicit half step
+    from ntamd import adi_coeffs
+    from ntamd.partition import PartitionPlan
+    pp = PartitionPlan(N, 16, adi_coeffs(r))
+    w = torch.empty_like(u)
+    tp = time_launches(lambda: pp.adi_half_x(u, w, r), 20, 3)
     return {"N": N, "r": r, "steps": nsteps, "s_per_run": t, "ms_per_step": 1e3 * t / nsteps,
             "effective_gbps": 2 * nsteps * half_bytes / t / 1e9,
+            "partitioned_x_half_step_16x128_us": tp["mean_s"] * 1e6,
             "note": "32 MiB field stays in the 256 MB Infinity Cache: effective, not HBM, GB/s"}
 
 
@@ -335,7 +343,8 @@ def main():
     if "adi" in cases:
         a = adi_case(2048, 0.5, 200, 3)
         extras["adi_2048_200steps_graph"] = a
-        log(f"[bench] ADI 2048^2: {a['ms_per_step']:.3f} ms/step")
+        log(f"[bench] ADI 2048^2: {a['ms_per_step']:.3f} ms/step; partitioned x half step "
+            f"(16 x 128 + reduced 32): {a['partitioned_x_half_step_16x128_us']:.1f} us")
     if "fp32_storage" in cases:
         extras["fp32_storage_1024"] = config4_case()
     if "batch" in cases:

@@ -63,6 +63,20 @@ __device__ __forceinline__ void st_from_double(T* p, double v) {
   *p = static_cast<T>(v);
 }
 
+// (A x)[e] of the level-0 near-Toeplitz matrix for row e of my chunk (left and
+// right: rows e-1 and e+1); the same expressions as in residual()
+__device__ __forceinline__ double apply_row(const NtPlanK& P, int e, int n, double left, double xe, double right) {
+  double ax;
+  if (e == 0) {
+    ax = P.b1 * xe + P.c1 * right;
+  } else if (e == n - 1) {
+    ax = P.an * left + P.bn * xe;
+  } else {
+    ax = P.ai * left + P.bi * xe + P.ci * right;
+  }
+  return ax;
+}
+
 // residual r = d - A x for the level-0 near-Toeplitz matrix, chunked layout,
 // formed in FP64 and rounded to the type of r (the FP32 correction solve takes
 // it as float, so no FP64 copy of r is kept)
@@ -90,29 +104,85 @@ __device__ __forceinline__ void residual(const double (&d)[R], const double (&x)
 }
 
 
-// Compact RHS (compact/rhs.py:21-36) of my chunk from the chunk values f, the
-// neighbour rows fl (row my_e0-1) and fr (row my_e0+R) and the six boundary
-// values bnd = f[0], f[1], f[2], f[n-1], f[n-2], f[n-3].  FULL (n == W*R):
-// only the first row of lane 0 and the last row of the last lane are closures.
+// Compact RHS (compact/rhs.py:21-36) of row k of my chunk from the chunk
+// values f (any floating type), the neighbour rows fl (row my_e0-1) and fr
+// (row my_e0+R), and the closures b0 (first row) and bn (last row).  FULL
+// (n == W*R): only the first row of lane 0 and the last row of the last lane
+// are closures.
+template <int R, bool FULL, typename TF>
+__device__ __forceinline__ double compact_rhs_elem(int k, const TF (&f)[R], double fl, double fr, double b0,
+                                                   double bn, int my_e0, int n, double s1) {
+  const int e = my_e0 + k;
+  const double left = (k > 0) ? static_cast<double>(f[k - 1]) : fl;
+  const double right = (k + 1 < R) ? static_cast<double>(f[k + 1]) : fr;
+  const double in_v = s1 * (right - left);
+  if constexpr (FULL) {
+    double v = in_v;
+    if (k == 0) v = (e == 0) ? b0 : v;
+    if (k == R - 1) v = (e == n - 1) ? bn : v;
+    return v;
+  } else {
+    return (e == 0) ? b0 : ((e == n - 1) ? bn : ((e < n) ? in_v : 0.0));
+  }
+}
+
+// closures b0 = s2 (-5 f0 + 4 f1 + f2), bn = -s2 (-5 f[n-1] + 4 f[n-2] + f[n-3])
+// from bnd = f[0], f[1], f[2], f[n-1], f[n-2], f[n-3]
+template <typename TB>
+__device__ __forceinline__ void compact_closures(const TB (&bnd)[6], double s2, double& b0, double& bn) {
+  b0 = s2 * (-5.0 * static_cast<double>(bnd[0]) + 4.0 * static_cast<double>(bnd[1]) + static_cast<double>(bnd[2]));
+  bn = -s2 * (-5.0 * static_cast<double>(bnd[3]) + 4.0 * static_cast<double>(bnd[4]) + static_cast<double>(bnd[5]));
+}
+
+// Compact RHS of my whole chunk (see compact_rhs_elem).
 template <int R, bool FULL = false>
 __device__ __forceinline__ void compact_rhs_chunk(const double (&f)[R], double fl, double fr,
                                                   const double (&bnd)[6], int my_e0, int n,
                                                   double s1, double s2, double (&d)[R]) {
-  const double b0 = s2 * (-5.0 * bnd[0] + 4.0 * bnd[1] + bnd[2]);
-  const double bn = -s2 * (-5.0 * bnd[3] + 4.0 * bnd[4] + bnd[5]);
+  double b0, bn;
+  compact_closures(bnd, s2, b0, bn);
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int e = my_e0 + k;
-    const double left = (k > 0) ? f[k - 1] : fl;
-    const double right = (k + 1 < R) ? f[k + 1] : fr;
-    const double in_v = s1 * (right - left);
-    if constexpr (FULL) {
-      double v = in_v;
-      if (k == 0) v = (e == 0) ? b0 : v;
-      if (k == R - 1) v = (e == n - 1) ? bn : v;
-      d[k] = v;
-    } else {
-      d[k] = (e == 0) ? b0 : ((e == n - 1) ? bn : ((e < n) ? in_v : 0.0));
+  for (int k = 0; k < R; ++k) d[k] = compact_rhs_elem<R, FULL>(k, f, fl, fr, b0, bn, my_e0, n, s1);
+}
+
+// Compact derivative of my chunk straight from FP32 chunk values f: the
+// right-hand side is formed in FP64 where it is used, so with FP32 + refinement
+// no FP64 copy of it stays live through the solves (it is recomputed from f
+// for each residual).  Gives the same bits as compact_rhs_chunk + solve_chunk.
+template <int R, int W, int PREC, bool FULL>
+__device__ __forceinline__ void compact_solve_from_f32(const float (&f)[R], const float (&bnd)[6], int my_e0,
+                                                       int n, double s1, double s2, double (&x)[R],
+                                                       const NtPlanK& P, int lane, int sweeps) {
+  const double fl = shup<W>(static_cast<double>(f[R - 1]), 1);
+  const double fr = shdn<W>(static_cast<double>(f[0]), 1);
+  double b0, bn;
+  compact_closures(bnd, s2, b0, bn);
+  if constexpr (PREC == PREC_F64) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[k] = compact_rhs_elem<R, FULL>(k, f, fl, fr, b0, bn, my_e0, n, s1);
+    cr_solve_regs<double, R, W, FULL>(x, P.d, P.nlev, lane);
+  } else {
+    float s[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      s[k] = static_cast<float>(compact_rhs_elem<R, FULL>(k, f, fl, fr, b0, bn, my_e0, n, s1));
+    cr_solve_regs<float, R, W, FULL>(s, P.f, P.nlev, lane);
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[k] = static_cast<double>(s[k]);
+    for (int it = 0; it < sweeps; ++it) {
+      const double xl = shup<W>(x[R - 1], 1);
+      const double xr = shdn<W>(x[0], 1);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int e = lane * R + k;
+        const double left = (k > 0) ? x[k - 1] : xl;
+        const double right = (k + 1 < R) ? x[k + 1] : xr;
+        const double dk = compact_rhs_elem<R, FULL>(k, f, fl, fr, b0, bn, my_e0, n, s1);
+        s[k] = static_cast<float>((e < n) ? dk - apply_row(P, e, n, left, x[k], right) : 0.0);
+      }
+      cr_solve_regs<float, R, W, FULL>(s, P.f, P.nlev, lane);
+#pragma unroll
+      for (int k = 0; k < R; ++k) x[k] += static_cast<double>(s[k]);
     }
   }
 }

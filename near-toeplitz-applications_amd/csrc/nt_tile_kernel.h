@@ -364,16 +364,10 @@ __global__ void __launch_bounds__(1024) line_kernel_tile_f32(const LineArgs args
       const int line = ql + h * HALF;            // tile-local line of this wavefront
       const bool active = (p0 + line) < nsys;    // uniform across the wavefront
       if (active) {
-        double f[R], bnd[6];
-#pragma unroll
-        for (int k = 0; k < R; ++k) f[k] = static_cast<double>(h == 0 ? fa[k] : fb[k]);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) bnd[i] = static_cast<double>(h == 0 ? ba[i] : bb[i]);
-        const double fl = shup<W>(f[R - 1], 1);
-        const double fr = shdn<W>(f[0], 1);
-        double d[R], x[R];
-        compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
-        solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
+        const float(&fh)[R] = (h == 0) ? fa : fb;
+        const float(&bh)[6] = (h == 0) ? ba : bb;
+        double x[R];
+        compact_solve_from_f32<R, W, PREC, FULL>(fh, bh, my_e0, n, args.s1, args.s2, x, P, lane, args.sweeps);
 #pragma unroll
         for (int k = 0; k < R; ++k)
           if (FULL || my_e0 + k < n) ldd[atd(ql, my_e0 + k)] = x[k];
