@@ -49,9 +49,9 @@ int g_tune[2][2][13][2] = {
     {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 1}, {1, 1}},
      {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 1}, {0, 1}}},
     // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
-    // 2 persistent whole-tile kernel
-    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 1}, {0, 0}, {2, 2}, {2, 2}, {2, 1}, {2, 1}},
-     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 2}, {2, 2}, {2, 2}, {1, 1}, {1, 1}}},
+    // 2 persistent whole-tile kernel, 3 the same with two lines per wavefront (two workgroups per CU)
+    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {2, 1}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 3}, {3, 3}, {3, 3}, {2, 2}, {1, 1}, {1, 1}}},
 };
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -160,7 +160,7 @@ extern "C" {
 
 const char* nt_last_error(void) { return g_err; }
 
-int nt_version(void) { return 100; }
+int nt_version(void) { return 101; }
 
 int nt_tune_set(int layout, int precision, int64_t n, int variant) {
   if (layout != 0 && layout != 1) return fail(NT_EINVAL, "nt_tune_set: layout must be 0 or 1");

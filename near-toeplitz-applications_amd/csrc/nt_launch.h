@@ -28,8 +28,10 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //           2: persistent whole-tile kernel, 16 lines (8 above 1024 points, and
 //              above 512 for ADI), next tile's loads in flight during this one;
 //              FP32-stored compact derivatives: 32 lines, two per wavefront
+//           3: as 2, with 32-line tiles (256-byte FP64 rows), two lines per
+//              wavefront, for lines up to 512 points outside ADI; otherwise as 2
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
-constexpr int kNumVariants[2] = {3, 3};
+constexpr int kNumVariants[2] = {3, 4};
 int tune_get(int layout, int prec, int n);
 void tune_set(int layout, int prec, int n, int variant);
 
@@ -97,28 +99,33 @@ inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) ==
 #define NT_WCT_WAVES_512 1
 #endif
 
+// Wavefronts per workgroup of the two-lines-per-wavefront tile kernel (variant 3)
+#ifndef NT_TILE2_WAVES
+#define NT_TILE2_WAVES 8
+#endif
+
 // Persistent whole-tile kernel for strided lines (nt_tile_kernel.h): a grid of
 // at most as many workgroups as fit on the device at once.
 int64_t persistent_grid(const void* kernel, int threads);
 
-#define NT_GOT(R_, WAVES_)                                                                 \
+#define NT_GOT(R_, WAVES_, LPW_)                                                           \
   do {                                                                                     \
     if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
     const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
-    a.tiles = (nsys_ + (WAVES_)-1) / (WAVES_);                                             \
+    a.tiles = (nsys_ + (WAVES_) * (LPW_)-1) / ((WAVES_) * (LPW_));                         \
     a.total = outer * a.tiles;                                                             \
     if (a.total <= 0) return hipSuccess;                                                   \
     const bool full_ = is_full(n, 64, R_);                                                 \
     const void* k_ = full_                                                                 \
-        ? reinterpret_cast<const void*>(&line_kernel_tile<TIn, TOut, R_, WAVES_, PROD, PREC, true>)  \
-        : reinterpret_cast<const void*>(&line_kernel_tile<TIn, TOut, R_, WAVES_, PROD, PREC, false>); \
+        ? reinterpret_cast<const void*>(&line_kernel_tile<TIn, TOut, R_, WAVES_, LPW_, PROD, PREC, true>)  \
+        : reinterpret_cast<const void*>(&line_kernel_tile<TIn, TOut, R_, WAVES_, LPW_, PROD, PREC, false>); \
     const int64_t cap_ = persistent_grid(k_, (WAVES_)*64);                                 \
     const int64_t grid_ = a.total < cap_ ? a.total : cap_;                                 \
     if (full_)                                                                             \
-      hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, PROD, PREC, true>),      \
+      hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, LPW_, PROD, PREC, true>),      \
                          dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
     else                                                                                   \
-      hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, PROD, PREC, false>),     \
+      hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, LPW_, PROD, PREC, false>),     \
                          dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
     return hipGetLastError();                                                              \
   } while (0)
@@ -181,7 +188,17 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     if (n <= 1024) NT_GO(16, 64, 4, 1024);
     if (n <= 2048) NT_GO(32, 64, 4, 2048);
   } else {
-    if (variant == 2) {
+    if constexpr (PROD != P_ADI && !(std::is_same_v<TIn, float> && std::is_same_v<TOut, double> &&
+                                     PROD == P_COMPACT)) {
+      if (variant == 3) {
+        // persistent whole-tile kernel, two lines per wavefront
+        if (n <= 64) NT_GOT(1, NT_TILE2_WAVES, 2);
+        if (n <= 128) NT_GOT(2, NT_TILE2_WAVES, 2);
+        if (n <= 256) NT_GOT(4, NT_TILE2_WAVES, 2);
+        if (n <= 512) NT_GOT(8, NT_TILE2_WAVES, 2);
+      }
+    }
+    if (variant >= 2) {
       // persistent whole-tile kernel, next tile prefetched into registers
       if constexpr (std::is_same_v<TIn, float> && std::is_same_v<TOut, double> && PROD == P_COMPACT) {
         // FP32 rows: 32-line tiles, two lines per wavefront (lines up to 1024 points)
@@ -190,20 +207,20 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
         if (n <= 256) NT_GOTF(4);
         if (n <= 512) NT_GOTF(8);
         if (n <= 1024) NT_GOTF(16);
-        if (n <= 2048) NT_GOT(32, 8);
+        if (n <= 2048) NT_GOT(32, 8, 1);
       } else if constexpr (PROD != P_ADI) {
-        if (n <= 64) NT_GOT(1, 16);
-        if (n <= 128) NT_GOT(2, 16);
-        if (n <= 256) NT_GOT(4, 16);
-        if (n <= 512) NT_GOT(8, 16);
-        if (n <= 1024) NT_GOT(16, 16);
-        if (n <= 2048) NT_GOT(32, 8);
+        if (n <= 64) NT_GOT(1, 16, 1);
+        if (n <= 128) NT_GOT(2, 16, 1);
+        if (n <= 256) NT_GOT(4, 16, 1);
+        if (n <= 512) NT_GOT(8, 16, 1);
+        if (n <= 1024) NT_GOT(16, 16, 1);
+        if (n <= 2048) NT_GOT(32, 8, 1);
       } else {
-        if (n <= 64) NT_GOT(1, 16);
-        if (n <= 128) NT_GOT(2, 16);
-        if (n <= 256) NT_GOT(4, 16);
-        if (n <= 512) NT_GOT(8, 16);
-        if (n <= 1024) NT_GOT(16, 8);
+        if (n <= 64) NT_GOT(1, 16, 1);
+        if (n <= 128) NT_GOT(2, 16, 1);
+        if (n <= 256) NT_GOT(4, 16, 1);
+        if (n <= 512) NT_GOT(8, 16, 1);
+        if (n <= 1024) NT_GOT(16, 8, 1);
         // 2048-point ADI lines do not fit a whole tile in LDS: blocked kernel below
       }
     }
@@ -231,5 +248,23 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
 
 #define NT_INSTANTIATE(TIn, TOut, LAYOUT, PROD, PREC) \
   template hipError_t launch_line<TIn, TOut, LAYOUT, PROD, PREC>(LineArgs&, int64_t, hipStream_t);
+
+// Each combination is instantiated (NT_INSTANTIATE) in exactly one nt_k_*.hip
+// file.  Everywhere else these declarations stop the implicit instantiation
+// that would compile every kernel a second time into the caller's code object.
+#define NT_DECLARE(TIn, TOut, LAYOUT, PROD, PREC) \
+  extern template hipError_t launch_line<TIn, TOut, LAYOUT, PROD, PREC>(LineArgs&, int64_t, hipStream_t);
+#define NT_DECLARE_BOTH(TIn, TOut, LAYOUT, PROD) \
+  NT_DECLARE(TIn, TOut, LAYOUT, PROD, PREC_F64) NT_DECLARE(TIn, TOut, LAYOUT, PROD, PREC_F32R)
+NT_DECLARE_BOTH(double, double, CONTIG, P_IDENT)
+NT_DECLARE_BOTH(double, double, STRIDED, P_IDENT)
+NT_DECLARE_BOTH(double, double, CONTIG, P_COMPACT)
+NT_DECLARE_BOTH(double, double, STRIDED, P_COMPACT)
+NT_DECLARE_BOTH(float, double, CONTIG, P_COMPACT)
+NT_DECLARE_BOTH(float, double, STRIDED, P_COMPACT)
+NT_DECLARE_BOTH(double, double, CONTIG, P_ADI)
+NT_DECLARE_BOTH(double, double, STRIDED, P_ADI)
+#undef NT_DECLARE_BOTH
+#undef NT_DECLARE
 
 }  // namespace nt

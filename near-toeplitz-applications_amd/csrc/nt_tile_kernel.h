@@ -1,15 +1,18 @@
 // Persistent whole-tile kernel for strided lines (y- and z-lines of a field).
 //
-// A tile is TS adjacent lines (one wavefront each, 64 lanes x R rows) whose
-// rows are contiguous runs of TS elements in memory.  The whole tile sits in
-// LDS at once, so a tile costs three workgroup barriers, not two per row
-// block.  The workgroup is persistent (the order in which it walks tiles is
-// described at the loop) and fetches the next tile's values into registers
+// A tile is TS adjacent lines (LPW per wavefront, solved in turn, 64 lanes x R
+// rows each) whose rows are contiguous runs of TS elements in memory.  The
+// whole tile sits in LDS at once, so a tile costs three workgroup barriers,
+// not two per row block.  With two lines per wavefront a 16-line tile needs
+// half the threads, so two workgroups share a CU.  The workgroup is persistent
+// (the order in which it walks tiles is described at tile_walk) and fetches
+// the next tile's values into registers
 // while the current tile is solved and stored (for R up to 16; longer lines
 // fetch after the solve, where the registers are free again).
 //
-//   fetch(t0) ; loop { barrier ; deposit(t) ; [fetch(next)] ; barrier ; pull
-//                      chunks ; solve ; barrier (ADI halo) ; write x ;
+//   fetch(t0) ; loop { barrier ; deposit(t) ; [fetch(next)] ; barrier ;
+//                      per line of the wavefront { pull chunks ; solve ;
+//                      barrier (ADI halo) ; write x } ;
 //                      barrier ; [fetch(next)] ; store tile }
 //
 // LDS holds ROWS x n doubles (plus chunk padding), which bounds n * ROWS; the
@@ -26,20 +29,67 @@
 #define NT_TILE_EARLY_MAX_R 16
 #endif
 
+// Tile walk order: 0 picks by layout (below), 1 always interleaves.
+#ifndef NT_TILE_ORDER
+#define NT_TILE_ORDER 0
+#endif
+// Contiguous ranges start at a per-workgroup offset of (workgroup * NT_TILE_ROT)
+// tiles, taken modulo the range length, and wrap around (0: no rotation).
+#ifndef NT_TILE_ROT
+#define NT_TILE_ROT 5
+#endif
+
 namespace nt {
 
-template <typename TIn, typename TOut, int R, int WAVES, int PROD, int PREC, bool FULL>
+// The tiles one workgroup of a persistent grid visits, tile(0) .. tile(count-1).
+// The launcher keeps gridDim.x <= total, so count >= 1.  Lines spread over
+// several slabs (y-lines: one slab per z-plane): a balanced contiguous range,
+// so the workgroups running together work on different slabs.  One slab
+// (z-lines, rows a whole plane apart): every gridDim.x-th tile, so the
+// workgroups running together read neighbouring pieces of the same rows.
+// Measured on MI355X, each order is the faster one for its case.
+struct TileWalk {
+  int64_t first, count, step, rot;
+  __device__ int64_t tile(int64_t k) const {
+    if (step != 1) return first + k * step;
+    const int64_t j = k + rot;
+    return first + (j < count ? j : j - count);
+  }
+};
+
+__device__ inline TileWalk tile_walk(int64_t total, bool ranges) {
+  const int64_t G = gridDim.x, w = blockIdx.x;
+  TileWalk tw;
+  if (ranges && NT_TILE_ORDER == 0) {
+    tw.first = (w * total) / G;
+    tw.count = ((w + 1) * total) / G - tw.first;
+    tw.step = 1;
+    tw.rot = (NT_TILE_ROT != 0 && tw.count > 0) ? (w * NT_TILE_ROT) % tw.count : 0;
+  } else {
+    tw.first = w;
+    tw.count = (total - w + G - 1) / G;
+    tw.step = G;
+    tw.rot = 0;
+  }
+  return tw;
+}
+
+template <typename TIn, typename TOut, int R, int WAVES, int LPW, int PROD, int PREC, bool FULL>
 __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs args_by_value) {
   (void)args_by_value;
   const LineArgs& args = kernel_args();
   constexpr int W = 64;
   constexpr int NT = WAVES * 64;
-  constexpr int TS = WAVES;                      // lines per tile, one per wavefront
+  constexpr int TS = WAVES * LPW;                // lines per tile
   constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
+  static_assert(LPW == 1 || HALO == 0, "ADI tiles hold one line per wavefront");
   constexpr int ROWS = TS + 2 * HALO;            // staged lines
   constexpr int NMAX = W * R;
-  constexpr int CP = (R >= 4) ? 2 : 0;
-  constexpr int EP = (NMAX / R) * (R + CP) + 2;  // LDS doubles per staged line
+  // Chunk padding: lane l's chunk starts (R + CP) doubles after lane l-1's, an
+  // odd count of 8-byte pairs of banks apart, so the chunk reads do not collide.
+  // Two lines per wavefront pad by one double (the 32-line tile must fit in LDS).
+  constexpr int CP = (LPW > 1) ? ((R >= 2) ? 1 : 0) : ((R >= 4) ? 2 : 0);
+  constexpr int EP = (NMAX / R) * (R + CP) + ((LPW > 1) ? 1 : 2);  // LDS doubles per staged line
   static_assert(ROWS * EP * 8 <= 160 * 1024, "tile does not fit in LDS");
   __shared__ __attribute__((aligned(16))) double lds[ROWS * EP];
 
@@ -58,7 +108,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
   // load, store and LDS offsets derived from it are recomputed for each tile
   // instead of being hoisted out of the tile loop and held (spilled) across it.
   int tid = threadIdx.x;
-  const int ql = threadIdx.x >> 6;               // my line in the tile
+  const int ql0 = threadIdx.x >> 6;              // my first line in the tile
   const int lane = threadIdx.x & 63;
   const int my_e0 = lane * R;
   const int64_t nsys = args.phys - 2 * HALO;
@@ -141,72 +191,64 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
   };
 
   constexpr bool EARLY = (R <= NT_TILE_EARLY_MAX_R);   // prefetch before the solve
-  // Tiles of this workgroup (the launcher keeps gridDim.x <= total, so none is
-  // empty).  Lines spread over several slabs (y-lines: one slab per z-plane):
-  // a balanced contiguous range, so the workgroups running together work on
-  // different slabs.  One slab (z-lines, rows a whole plane apart): every
-  // gridDim.x-th tile, so the workgroups running together read neighbouring
-  // pieces of the same rows.  Measured on MI355X, each order is the faster one
-  // for its case.
-  const bool ranges = args.ostride != 0;
-  const int64_t t_begin = ranges ? (static_cast<int64_t>(blockIdx.x) * total) / gridDim.x
-                                 : static_cast<int64_t>(blockIdx.x);
-  const int64_t t_end = ranges ? ((static_cast<int64_t>(blockIdx.x) + 1) * total) / gridDim.x : total;
-  const int64_t t_step = ranges ? 1 : static_cast<int64_t>(gridDim.x);
-  int64_t t = t_begin;
-  Tile g = geometry(t);
+  const TileWalk walk = tile_walk(total, args.ostride != 0);
+  int64_t k = 0;
+  Tile g = geometry(walk.tile(0));
   fetch(g);
   for (;;) {
     tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);   // the mask keeps its range known
     __syncthreads();                             // the previous tile's stores have read the LDS
     deposit(g);
-    const int64_t tn = t + t_step;
-    const bool more = tn < t_end;                // uniform
+    const bool more = k + 1 < walk.count;        // uniform
     Tile gn = g;
-    if (more) gn = geometry(tn);
+    if (more) gn = geometry(walk.tile(k + 1));
     if (EARLY && more) fetch(gn);                // the next tile's loads fly during this one
     __syncthreads();
 
-    const bool active = (g.p0 + ql) < nsys;      // uniform across the wavefront
-    double x[R];
-    if (active) {
-      double f[R];
+#pragma unroll 1
+    for (int h = 0; h < LPW; ++h) {
+      const int ql = ql0 + h * WAVES;            // tile-local line of this wavefront
+      const bool active = (g.p0 + ql) < nsys;    // uniform across the wavefront
+      double x[R];
+      if (active) {
+        double f[R];
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const int e = my_e0 + k;
-        double v = 0.0;
-        if (FULL || e < n) {
-          const double c = lds[at(ql + HALO, e)];
-          if (PROD == P_ADI) {
-            v = (e == 0 || e == n - 1) ? c : args.s1 * (lds[at(ql, e)] + lds[at(ql + 2, e)]) + args.s2 * c;
-          } else {
-            v = c;
+        for (int k = 0; k < R; ++k) {
+          const int e = my_e0 + k;
+          double v = 0.0;
+          if (FULL || e < n) {
+            const double c = lds[at(ql + HALO, e)];
+            if (PROD == P_ADI) {
+              v = (e == 0 || e == n - 1) ? c : args.s1 * (lds[at(ql, e)] + lds[at(ql + 2, e)]) + args.s2 * c;
+            } else {
+              v = c;
+            }
           }
+          f[k] = v;
         }
-        f[k] = v;
+        double d[R];
+        if (PROD == P_COMPACT) {
+          double bnd[6];
+#pragma unroll
+          for (int i = 0; i < 6; ++i) bnd[i] = lds[at(ql, (i < 3) ? i : n + 2 - i)];
+          const double fl = shup<W>(f[R - 1], 1);
+          const double fr = shdn<W>(f[0], 1);
+          compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
+        } else {
+#pragma unroll
+          for (int k = 0; k < R; ++k) d[k] = f[k];
+        }
+        solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
       }
-      double d[R];
-      if (PROD == P_COMPACT) {
-        double bnd[6];
+      // the solution overwrites the rows just read: the wavefront's own line, and
+      // with a halo also the neighbours' lines, which other wavefronts read
+      if (HALO) __syncthreads();
+      else wave_lds_sync();
+      if (active) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) bnd[i] = lds[at(ql, (i < 3) ? i : n + 2 - i)];
-        const double fl = shup<W>(f[R - 1], 1);
-        const double fr = shdn<W>(f[0], 1);
-        compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
-      } else {
-#pragma unroll
-        for (int k = 0; k < R; ++k) d[k] = f[k];
+        for (int k = 0; k < R; ++k)
+          if (FULL || my_e0 + k < n) lds[at(ql + HALO, my_e0 + k)] = x[k];
       }
-      solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
-    }
-    // the solution overwrites the rows just read: the wavefront's own line, and
-    // with a halo also the neighbours' lines, which other wavefronts read
-    if (HALO) __syncthreads();
-    else wave_lds_sync();
-    if (active) {
-#pragma unroll
-      for (int k = 0; k < R; ++k)
-        if (FULL || my_e0 + k < n) lds[at(ql + HALO, my_e0 + k)] = x[k];
     }
     __syncthreads();
     if (!EARLY && more) fetch(gn);               // in flight during this tile's stores
@@ -238,7 +280,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
       }
     }
     if (!more) break;
-    t = tn;
+    ++k;
     g = gn;
   }
 }
@@ -286,13 +328,10 @@ __global__ void __launch_bounds__(1024) line_kernel_tile_f32(const LineArgs args
   auto atf = [&](int row, int el) -> int { return row * EPF + el + el / R; };
   auto atd = [&](int row, int el) -> int { return row * EPD + el + (el / R) * CP; };
 
-  const bool ranges = args.ostride != 0;         // tile order as in line_kernel_tile
-  const int64_t t_begin = ranges ? (static_cast<int64_t>(blockIdx.x) * total) / gridDim.x
-                                 : static_cast<int64_t>(blockIdx.x);
-  const int64_t t_end = ranges ? ((static_cast<int64_t>(blockIdx.x) + 1) * total) / gridDim.x : total;
-  const int64_t t_step = ranges ? 1 : static_cast<int64_t>(gridDim.x);
+  const TileWalk walk = tile_walk(total, args.ostride != 0);
 
-  for (int64_t t = t_begin; t < t_end; t += t_step) {
+  for (int64_t k = 0; k < walk.count; ++k) {
+    const int64_t t = walk.tile(k);
     tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);
     const int64_t o = t / args.tiles;
     const int64_t p0 = (t - o * args.tiles) * TS;
