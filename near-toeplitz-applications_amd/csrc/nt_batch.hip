@@ -29,7 +29,12 @@ constexpr int BATCH_TS = 16;                // lines per tile
 constexpr int BATCH_WAVES = 8;              // wavefronts per workgroup
 constexpr int BATCH_LPW = BATCH_TS / BATCH_WAVES;   // lines solved by each wavefront
 constexpr int BATCH_NT = BATCH_WAVES * 64;
-constexpr int BATCH_EP = BATCH_MAXN + 2;    // LDS stride between lines (doubles)
+// LDS row of a line: point e of a line with R = 2^rsel rows per lane sits at
+// e + (e >> rsel), i.e. one double of padding after each lane's chunk, so a
+// lane's chunk starts R + 1 doubles after its neighbour's (an odd number of
+// 8-byte bank pairs for R >= 2): the solve's chunk reads and writes do not
+// collide.  A row then holds at most 64 (R + 1) doubles.
+constexpr int BATCH_EP = BATCH_MAXN + BATCH_MAXN / 8 + 2;   // LDS stride between lines (doubles)
 // every thread moves BATCH_TS * BATCH_MAXN / BATCH_NT elements of a tile
 constexpr int BATCH_PER = BATCH_TS * BATCH_MAXN / BATCH_NT;
 
@@ -78,7 +83,7 @@ __device__ __forceinline__ void tile_slot(int layout, int tid, int k, int& line,
 
 template <typename TIn>
 __device__ __forceinline__ void stage_tile(double* lds, const TIn* __restrict__ in, int layout, int tid, int rows,
-                                           int n, int64_t base, int64_t pstride, int64_t estride) {
+                                           int n, int rsel, int64_t base, int64_t pstride, int64_t estride) {
   double v[BATCH_PER];
   // issue every load of the tile before the first LDS write so they overlap
 #pragma unroll
@@ -93,25 +98,31 @@ __device__ __forceinline__ void stage_tile(double* lds, const TIn* __restrict__ 
   for (int k = 0; k < BATCH_PER; ++k) {
     int line, e;
     tile_slot(layout, tid, k, line, e);
-    if (line < rows && e < n) lds[line * BATCH_EP + e] = v[k];
+    if (line < rows && e < n) lds[line * BATCH_EP + e + (e >> rsel)] = v[k];
   }
 }
 
-// One line held in LDS row `row` (n points): lane l takes points l R .. l R + R - 1,
-// builds the right-hand side (compact producer) or takes the values as they are
-// (solve), runs the register CR and writes the solution back over the row.
+// One line held in LDS row `row` (n points, padded as described at BATCH_EP):
+// lane l takes points l R .. l R + R - 1, builds the right-hand side (compact
+// producer) or takes the values as they are (solve), runs the register CR and
+// writes the solution back over the row.
 template <int R, bool FULL>
 __device__ __forceinline__ void solve_line(double* row, int n, int prod, int prec, const NtPlanK& P, double s1,
                                            double s2, int sweeps, int lane) {
+  constexpr int RS = (R == 1) ? 0 : (R == 2) ? 1 : (R == 4) ? 2 : 3;
   const int my_e0 = lane * R;
+  double* chunk = row + lane * (R + 1);
   double f[R];
 #pragma unroll
-  for (int k = 0; k < R; ++k) f[k] = (FULL || my_e0 + k < n) ? row[my_e0 + k] : 0.0;
+  for (int k = 0; k < R; ++k) f[k] = (FULL || my_e0 + k < n) ? chunk[k] : 0.0;
   double d[R];
   if (prod == P_COMPACT) {
     double bnd[6];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) bnd[i] = row[(i < 3) ? i : n + 2 - i];
+    for (int i = 0; i < 6; ++i) {
+      const int e = (i < 3) ? i : n + 2 - i;
+      bnd[i] = row[e + (e >> RS)];
+    }
     const double fl = shup<64>(f[R - 1], 1);
     const double fr = shdn<64>(f[0], 1);
     compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, s1, s2, d);
@@ -124,7 +135,7 @@ __device__ __forceinline__ void solve_line(double* row, int n, int prod, int pre
   else solve_chunk<R, 64, PREC_F32R, FULL>(d, x, P, lane, sweeps);
 #pragma unroll
   for (int k = 0; k < R; ++k)
-    if (FULL || my_e0 + k < n) row[my_e0 + k] = x[k];
+    if (FULL || my_e0 + k < n) chunk[k] = x[k];
 }
 
 __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
@@ -167,19 +178,20 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
     const int64_t pstride = J.pstride, estride = J.estride;
     double* __restrict__ out = static_cast<double*>(J.out);
 
+    // rows per lane R = 2^rsel follow the line length (64 lanes x R >= n)
+    const int rsel = (n <= 64) ? 0 : (n <= 128) ? 1 : (n <= 256) ? 2 : 3;
+
     // ---- stage the tile (TS lines x n) through LDS, coalesced along memory ------
     if (uni(J.in_f32))
-      stage_tile(lds, static_cast<const float*>(J.in), layout, tid, rows, n, base, pstride, estride);
+      stage_tile(lds, static_cast<const float*>(J.in), layout, tid, rows, n, rsel, base, pstride, estride);
     else
-      stage_tile(lds, static_cast<const double*>(J.in), layout, tid, rows, n, base, pstride, estride);
+      stage_tile(lds, static_cast<const double*>(J.in), layout, tid, rows, n, rsel, base, pstride, estride);
     __syncthreads();
 
     // ---- each wavefront solves its lines; a line's LDS row is read and then
     // overwritten by the same wavefront only, so no barrier in between.  The
-    // rows per lane follow the line length (64 lanes x R >= n), with the
-    // compile-time level structure when n == 64 R. -------------------------------
+    // compile-time level structure applies when n == 64 R. -----------------------
     const NtPlanK& P = s_plan;
-    const int rsel = (n <= 64) ? 0 : (n <= 128) ? 1 : (n <= 256) ? 2 : 3;
     const bool full = (n == (64 << rsel));
 #pragma unroll 1
     for (int h = 0; h < BATCH_LPW; ++h) {
@@ -210,7 +222,7 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
       int line, e;
       tile_slot(layout, tid, k, line, e);
       if (line < rows && e < n)
-        __builtin_nontemporal_store(lds[line * BATCH_EP + e], out + base + line * pstride + e * estride);
+        __builtin_nontemporal_store(lds[line * BATCH_EP + e + (e >> rsel)], out + base + line * pstride + e * estride);
     }
     __syncthreads();   // LDS, the plan copy and s_tile are reused by the next tile
   }

@@ -286,6 +286,129 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
 }
 
 
+// Fast path of line_kernel_tile for the common case: FP64 in and out, n == 64 R
+// a power of two, every tile whole and 16-byte aligned (the launcher checks),
+// no halo.  Every load and store is unconditional, with compile-time counts,
+// and the loop is rotated so that the deposit of the next tile follows this
+// tile's stores on the only path into it.  A wavefront's vector-memory counter
+// covers loads and stores alike; here the compiler can wait for the prefetched
+// loads alone (they were issued before this tile's stores) instead of for
+// everything, so the stores drain while the next tile is deposited and
+// solved.  The last tile is peeled.
+//
+//   fetch(t0) ; deposit ; barrier ; loop { fetch(next) ; solve ; barrier ;
+//              store ; barrier ; deposit(next) ; barrier } ; solve ; barrier ; store
+template <int R, int WAVES, int LPW, int PROD, int PREC>
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
+  static_assert(PROD != P_ADI, "ADI tiles carry halo lines");
+  constexpr int W = 64;
+  constexpr int NT = WAVES * 64;
+  constexpr int TS = WAVES * LPW;                // lines per tile
+  constexpr int N = W * R;                       // line length
+  constexpr int CP = (LPW > 1) ? ((R >= 2) ? 1 : 0) : ((R >= 4) ? 2 : 0);   // as line_kernel_tile
+  constexpr int EP = (N / R) * (R + CP) + ((LPW > 1) ? 1 : 2);
+  static_assert(TS * EP * 8 <= 160 * 1024, "tile does not fit in LDS");
+  __shared__ __attribute__((aligned(16))) double lds[TS * EP];
+  constexpr int VPR = TS / 2;                    // 16-byte vectors per row
+  static_assert((VPR * N) % NT == 0, "every thread moves the same number of vectors");
+  constexpr int PFV = VPR * N / NT;              // vectors per thread per tile
+
+  const NtPlanK& P = args.plan;
+  int tid = threadIdx.x;                         // refreshed per tile (see line_kernel_tile)
+  const int ql0 = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int my_e0 = lane * R;
+  const int64_t estride = args.estride;
+  auto at = [&](int row, int el) -> int { return row * EP + el + (el / R) * CP; };
+  auto base_of = [&](int64_t t) -> int64_t {
+    const int64_t o = t / args.tiles;
+    return o * args.ostride + (t - o * args.tiles) * TS;
+  };
+
+  nt_dbl2 pf[PFV];
+  auto fetch = [&](int64_t base) {
+    const double* __restrict__ in = static_cast<const double*>(args.in) + base;
+#pragma unroll
+    for (int i = 0; i < PFV; ++i) {
+      const int idx = tid + i * NT;
+      const int vr = idx % VPR, el = idx / VPR;
+      pf[i] = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(in + el * estride + vr * 2));
+    }
+  };
+  auto deposit = [&]() {
+#pragma unroll
+    for (int i = 0; i < PFV; ++i) {
+      const int idx = tid + i * NT;
+      const int vr = idx % VPR, el = idx / VPR;
+      lds[at(vr * 2, el)] = pf[i].x;
+      lds[at(vr * 2 + 1, el)] = pf[i].y;
+    }
+  };
+  auto solve_tile = [&]() {
+#pragma unroll 1
+    for (int h = 0; h < LPW; ++h) {
+      const int ql = ql0 + h * WAVES;            // tile-local line of this wavefront
+      double f[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) f[k] = lds[at(ql, my_e0 + k)];
+      double d[R];
+      if (PROD == P_COMPACT) {
+        double bnd[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bnd[i] = lds[at(ql, (i < 3) ? i : N + 2 - i)];
+        const double fl = shup<W>(f[R - 1], 1);
+        const double fr = shdn<W>(f[0], 1);
+        compact_rhs_chunk<R, true>(f, fl, fr, bnd, my_e0, N, args.s1, args.s2, d);
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) d[k] = f[k];
+      }
+      double x[R];
+      solve_chunk<R, W, PREC, true>(d, x, P, lane, args.sweeps);
+      wave_lds_sync();                           // the solution overwrites the line just read
+#pragma unroll
+      for (int k = 0; k < R; ++k) lds[at(ql, my_e0 + k)] = x[k];
+    }
+  };
+  auto store = [&](int64_t base) {
+    double* __restrict__ out = static_cast<double*>(args.out) + base;
+#pragma unroll
+    for (int i = 0; i < PFV; ++i) {
+      const int idx = tid + i * NT;
+      const int vr = idx % VPR, el = idx / VPR;
+      nt_dbl2 v;
+      v.x = lds[at(vr * 2, el)];
+      v.y = lds[at(vr * 2 + 1, el)];
+      __builtin_nontemporal_store(v, reinterpret_cast<nt_dbl2*>(out + el * estride + vr * 2));
+    }
+  };
+
+  const TileWalk walk = tile_walk(args.total, args.ostride != 0);
+  int64_t b = base_of(walk.tile(0));
+  fetch(b);
+  deposit();
+  __syncthreads();
+  for (int64_t k = 0; k + 1 < walk.count; ++k) {
+    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);   // the mask keeps its range known
+    const int64_t bn = base_of(walk.tile(k + 1));
+    fetch(bn);                                   // in flight during this tile's solve and stores
+    solve_tile();
+    __syncthreads();
+    store(b);
+    __syncthreads();                             // the stores have read the LDS
+    deposit();
+    __syncthreads();
+    b = bn;
+  }
+  tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);
+  solve_tile();
+  __syncthreads();
+  store(b);
+}
+
+
 // FP32-stored fields (compact derivative, FP64 result): a 16-line tile of FP32
 // rows reads 64-byte runs, half of what a row of FP64 gives.  This variant
 // takes 32-line tiles (128-byte FP32 runs), two lines per wavefront: the tile
