@@ -29,7 +29,8 @@
 #define NT_TILE_EARLY_MAX_R 16
 #endif
 
-// Tile walk order: 0 picks by layout (below), 1 always interleaves.
+// Tile walk order: 0 picks by layout (below), 1 always interleaves, 2 always
+// takes contiguous ranges.
 #ifndef NT_TILE_ORDER
 #define NT_TILE_ORDER 0
 #endif
@@ -60,7 +61,7 @@ struct TileWalk {
 __device__ inline TileWalk tile_walk(int64_t total, bool ranges) {
   const int64_t G = gridDim.x, w = blockIdx.x;
   TileWalk tw;
-  if (ranges && NT_TILE_ORDER == 0) {
+  if ((ranges && NT_TILE_ORDER == 0) || NT_TILE_ORDER == 2) {
     tw.first = (w * total) / G;
     tw.count = ((w + 1) * total) / G - tw.first;
     tw.step = 1;
@@ -385,7 +386,10 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
     }
   };
 
-  const TileWalk walk = tile_walk(args.total, args.ostride != 0);
+  // Contiguous tile ranges for both axes here.  With this kernel, z-lines too
+  // are faster that way (512^3 d/dz: 492-544 us against 510-578 us for the
+  // interleaved order, over nine placements of the output relative to the input).
+  const TileWalk walk = tile_walk(args.total, true);
   int64_t b = base_of(walk.tile(0));
   fetch(b);
   deposit();
