@@ -112,6 +112,13 @@ inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) ==
 #ifndef NT_TILE_FAST
 #define NT_TILE_FAST 1
 #endif
+// Largest rows per lane of the FP64 fast path.  At 1024 points (R = 16, the
+// prefetch split around the solve, no spills) it was measured on 1024^3 d/dz
+// with five placements of the output: 3.9-4.0 ms at three, 5.6-6.0 ms at the
+// other two, where the general kernel takes 4.3-4.5 ms at all five.  So 8.
+#ifndef NT_TILE_FAST_MAX_R
+#define NT_TILE_FAST_MAX_R 8
+#endif
 
 // Whole 16-byte aligned tiles of `ts` adjacent strided lines only: the fast
 // tile kernel's loads and stores are all unconditional 16-byte accesses.
@@ -135,7 +142,7 @@ int64_t persistent_grid(const void* kernel, int threads);
     const bool full_ = is_full(n, 64, R_);                                                 \
     if constexpr (NT_TILE_FAST && std::is_same_v<TIn, double> && std::is_same_v<TOut, double> &&     \
                   PROD != P_ADI && ((R_) * (LPW_)) % 2 == 0 &&                             \
-                  (R_) <= (PREC == PREC_F64 || (LPW_) == 1 ? 8 : 4)) {                     \
+                  (R_) <= (PREC == PREC_F64 ? NT_TILE_FAST_MAX_R : ((LPW_) == 1 ? 8 : 4))) { \
       if (full_ && tile_fast_ok(a, (WAVES_) * (LPW_))) {                                   \
         const void* kf_ = reinterpret_cast<const void*>(&line_kernel_tile_fast<R_, WAVES_, LPW_, PROD, PREC>); \
         const int64_t capf_ = persistent_grid(kf_, (WAVES_)*64);                           \

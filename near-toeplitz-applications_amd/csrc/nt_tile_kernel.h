@@ -315,6 +315,10 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
   constexpr int VPR = TS / 2;                    // 16-byte vectors per row
   static_assert((VPR * N) % NT == 0, "every thread moves the same number of vectors");
   constexpr int PFV = VPR * N / NT;              // vectors per thread per tile
+  // Long lines (R = 16) fetch the first half of the next tile before the solve
+  // and the second half after it, still ahead of this tile's stores: half the
+  // prefetch registers are live during the solve.
+  constexpr int PF_EARLY = (R >= 16) ? PFV / 2 : PFV;
 
   const NtPlanK& P = args.plan;
   int tid = threadIdx.x;                         // refreshed per tile (see line_kernel_tile)
@@ -329,10 +333,10 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
   };
 
   nt_dbl2 pf[PFV];
-  auto fetch = [&](int64_t base) {
+  auto fetch = [&](int64_t base, int i0, int i1) {
     const double* __restrict__ in = static_cast<const double*>(args.in) + base;
 #pragma unroll
-    for (int i = 0; i < PFV; ++i) {
+    for (int i = i0; i < i1; ++i) {
       const int idx = tid + i * NT;
       const int vr = idx % VPR, el = idx / VPR;
       pf[i] = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(in + el * estride + vr * 2));
@@ -391,14 +395,15 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
   // interleaved order, over nine placements of the output relative to the input).
   const TileWalk walk = tile_walk(args.total, true);
   int64_t b = base_of(walk.tile(0));
-  fetch(b);
+  fetch(b, 0, PFV);
   deposit();
   __syncthreads();
   for (int64_t k = 0; k + 1 < walk.count; ++k) {
     tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);   // the mask keeps its range known
     const int64_t bn = base_of(walk.tile(k + 1));
-    fetch(bn);                                   // in flight during this tile's solve and stores
+    fetch(bn, 0, PF_EARLY);                      // in flight during this tile's solve and stores
     solve_tile();
+    fetch(bn, PF_EARLY, PFV);
     __syncthreads();
     store(b);
     __syncthreads();                             // the stores have read the LDS
