@@ -102,7 +102,7 @@ def cpu_baseline(N, sample_lines, reps=3):
         blas_threads = None
     return {"value": sample_lines * N / med, "unit": "grid pts/s", "cores": 1,
             "kind": "port",
-            "sample": f"{sample_lines} random x-lines of the 512^3 d/dx workload "
+            "sample": f"{sample_lines} x-lines of the 512^3 d/dx shape, y and z drawn at random "
                       f"({sample_lines * N} pts), NumPy RHS + one scipy.linalg.solve_banded "
                       f"call (LAPACK gtsv, single-threaded), median of {reps}",
             "seconds": med, "host_cores_available": len(os.sched_getaffinity(0)),
@@ -360,7 +360,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (SciPy banded) ...")
-        cpu = cpu_baseline(N, 131072)
+        cpu = cpu_baseline(N, 262144)     # every x-line of the 512^3 workload, ~3 s per repetition
         log(f"[bench] CPU: {cpu['value']:.3e} pts/s")
 
     pts = N ** 3
