@@ -239,29 +239,46 @@ int nt_part_solve_lines(const nt_part_plan* pp, int producer, const void* in, vo
   }
   if (grid <= 0) return nt::set_error(NT_OK, "%s", "");
   if (grid > 0x7fffffffLL) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: too many lines");
+  // Segment of W lanes x R rows per subdomain (W R >= m).  Many subdomains per
+  // line (P >= 12) or short ones (m <= 64): 16-lane segments (four subdomains per
+  // wavefront, R <= 8) up to 128 points, 32 lanes up to 256, fewer wavefronts per
+  // line.  Otherwise one 64-lane wavefront per subdomain: the first and last
+  // subdomains have a wavefront each, so at small P packed segments would sit
+  // idle.  Measured on MI355X: P = 16, m = 128: 0.296 ms (segments) against
+  // 0.400 ms on 128^2 lines; P = 4, m = 128: 0.132 ms against 0.104 ms.
   const int m = pp->m;
-  int R = 1;
-  while (64 * R < m) R *= 2;
-  const dim3 g(static_cast<unsigned>(grid)), b(64 * pp->P);
+  int R = 1, W = 64;
+  if ((pp->P >= 12 || m <= 64) && m <= 32 * 8) {
+    W = (m <= 16 * 8) ? 16 : 32;
+    while (W * R < m) R *= 2;
+  } else {
+    while (W * R < m) R *= 2;
+  }
+  const dim3 g(static_cast<unsigned>(grid)), b(64 * nt::part_waves(pp->P, W));
   hipStream_t st = static_cast<hipStream_t>(stream);
-#define NT_PGO(RR)                                                                       \
-  case RR:                                                                               \
+#define NT_PGO(RR, WW)                                                                   \
+  if (R == (RR) && W == (WW)) {                                                          \
+    if (static_cast<int>(b.x) > nt::part_max_threads<RR>())                          \
+      return nt::set_error(NT_EUNSUPPORTED, "%s", "nt_part_solve_lines: too many subdomains"); \
     if (producer == 0)                                                                   \
-      hipLaunchKernelGGL((nt::part_kernel<RR, nt::P_IDENT>), g, b, 0, st, a);            \
+      hipLaunchKernelGGL((nt::part_kernel<RR, WW, nt::P_IDENT>), g, b, 0, st, a);        \
     else if (producer == 1)                                                              \
-      hipLaunchKernelGGL((nt::part_kernel<RR, nt::P_COMPACT>), g, b, 0, st, a);          \
+      hipLaunchKernelGGL((nt::part_kernel<RR, WW, nt::P_COMPACT>), g, b, 0, st, a);      \
     else                                                                                 \
-      hipLaunchKernelGGL((nt::part_kernel<RR, nt::P_ADI>), g, b, 0, st, a);              \
-    break;
-  switch (R) {
-    NT_PGO(1)
-    NT_PGO(2)
-    NT_PGO(4)
-    NT_PGO(8)
-    NT_PGO(16)
-    NT_PGO(32)
-    default:
-      return nt::set_error(NT_EUNSUPPORTED, "%s", "nt_part_solve_lines: subdomain too large");
+      hipLaunchKernelGGL((nt::part_kernel<RR, WW, nt::P_ADI>), g, b, 0, st, a);          \
+  } else
+  NT_PGO(1, 16)
+  NT_PGO(2, 16)
+  NT_PGO(4, 16)
+  NT_PGO(8, 16)
+  NT_PGO(8, 32)
+  NT_PGO(1, 64)
+  NT_PGO(2, 64)
+  NT_PGO(4, 64)
+  NT_PGO(8, 64)
+  NT_PGO(16, 64)
+  NT_PGO(32, 64) {
+    return nt::set_error(NT_EUNSUPPORTED, "%s", "nt_part_solve_lines: subdomain too large");
   }
 #undef NT_PGO
   const hipError_t e = hipGetLastError();

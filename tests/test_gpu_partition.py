@@ -19,8 +19,11 @@ def rel_err(x, ref):
 
 @pytest.mark.parametrize("coeffs", [COMPACT, adi.pr_coeffs(0.5), (2.0, 0.3, 0.5, 3.0, 0.7, 0.2, 1.7)])
 @pytest.mark.parametrize("n,P", [(2048, 16), (512, 4), (96, 3), (64, 1), (100, 2), (1024, 8),
-                                 (2000, 16)])
+                                 (2000, 16), (512, 2), (1536, 8), (2048, 4), (1000, 2), (2048, 2),
+                                 (1800, 3), (2048, 1), (64, 16), (34, 2), (1812, 12)])
 def test_partitioned_equals_monolithic(dev, coeffs, n, P):
+    """Subdomains of m = n / P points on segments of 16, 32 or 64 lanes (four, two
+    or one per wavefront; 4..2048 points, R = 1..32), idle segments included."""
     from ntamd.partition import PartitionPlan
     d = np.random.default_rng(n + P).standard_normal((33, n))
     x = torch.tensor(d, device=dev)
