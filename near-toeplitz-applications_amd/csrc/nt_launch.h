@@ -24,7 +24,8 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //           2: workgroup tile of 4 lines staged through LDS (any n, alignment)
 //   STRIDED 0: tile of 16 adjacent lines, 64 lanes per line (1024 threads)
 //           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads);
-//              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads)
+//              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads),
+//              for ADI above 1024 points the whole tile in LDS, RHS formed at load
 //           2: persistent whole-tile kernel, 16 lines (8 above 1024 points, and
 //              above 512 for ADI), next tile's loads in flight during this one;
 //              FP32-stored compact derivatives: 32 lines, two per wavefront
@@ -166,6 +167,28 @@ int64_t persistent_grid(const void* kernel, int threads);
     return hipGetLastError();                                                              \
   } while (0)
 
+// ADI along strided lines with the right-hand side formed at load (nt_tile_kernel.h)
+#define NT_GOTA(R_, WAVES_)                                                                \
+  do {                                                                                     \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    const int64_t nsys_ = a.phys - 2;                                                      \
+    a.tiles = (nsys_ + (WAVES_)-1) / (WAVES_);                                             \
+    a.total = outer * a.tiles;                                                             \
+    if (a.total <= 0) return hipSuccess;                                                   \
+    const bool full_ = is_full(n, 64, R_);                                                 \
+    const void* k_ = full_ ? reinterpret_cast<const void*>(&line_kernel_tile_adi<R_, WAVES_, PREC, true>)   \
+                           : reinterpret_cast<const void*>(&line_kernel_tile_adi<R_, WAVES_, PREC, false>); \
+    const int64_t cap_ = persistent_grid(k_, (WAVES_)*64);                                 \
+    const int64_t grid_ = a.total < cap_ ? a.total : cap_;                                 \
+    if (full_)                                                                             \
+      hipLaunchKernelGGL((line_kernel_tile_adi<R_, WAVES_, PREC, true>), dim3(static_cast<unsigned>(grid_)), \
+                         dim3((WAVES_)*64), 0, st, a);                                     \
+    else                                                                                   \
+      hipLaunchKernelGGL((line_kernel_tile_adi<R_, WAVES_, PREC, false>), dim3(static_cast<unsigned>(grid_)), \
+                         dim3((WAVES_)*64), 0, st, a);                                     \
+    return hipGetLastError();                                                              \
+  } while (0)
+
 // FP32-stored compact derivative along strided lines: 32-line tiles (nt_tile_kernel.h)
 #define NT_GOTF(R_)                                                                        \
   do {                                                                                     \
@@ -257,10 +280,17 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
         if (n <= 256) NT_GOT(4, 16, 1);
         if (n <= 512) NT_GOT(8, 16, 1);
         if (n <= 1024) NT_GOT(16, 8, 1);
-        // 2048-point ADI lines do not fit a whole tile in LDS: blocked kernel below
+        // 2048-point ADI lines with their halo lines do not fit a whole tile in LDS:
+        // the right-hand side is formed while loading instead
+        if (n <= 2048) NT_GOTA(32, 8);
       }
     }
     if (variant == 1) {
+      // ADI on 1025..2048-point lines: whole 8-line tiles with the right-hand side
+      // formed at load (ADI 2048^2 step 67.8 -> 60.0 us against the blocked kernel)
+      if constexpr (PROD == P_ADI) {
+        if (n > 1024 && n <= 2048) NT_GOTA(32, 8);
+      }
       // 32 adjacent lines per tile (256-byte FP64 rows), 32 lanes per line
       if (n <= 32) NT_GOS(1, 32, 16, 32);
       if (n <= 64) NT_GOS(2, 32, 16, 64);

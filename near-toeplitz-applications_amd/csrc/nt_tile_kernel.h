@@ -418,6 +418,102 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
 }
 
 
+// ADI half steps along strided lines too long for line_kernel_tile's halo rows
+// (2048 points: 8 lines plus 2 halo lines exceed the LDS).  The right-hand side
+// is the explicit stencil across lines, s1 (u[j-1] + u[j+1]) + s2 u[j] with
+// identity end rows, so it needs the neighbouring lines only at the same
+// position: it is formed while the tile is loaded (each element's three values
+// come in with scalar loads that share cache lines), and only the tile's own
+// lines sit in LDS.  One line per wavefront; persistent over tiles.
+//
+//   loop { barrier ; load + form RHS into LDS (batches of BATCH elements per
+//          thread, their 3 BATCH loads in flight together) ; barrier ; pull
+//          chunk ; solve ; write x ; barrier ; store tile }
+template <int R, int WAVES, int PREC, bool FULL>
+__global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_adi(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
+  constexpr int W = 64;
+  constexpr int NT = WAVES * 64;
+  constexpr int TS = WAVES;                      // lines per tile, one per wavefront
+  constexpr int NMAX = W * R;
+  constexpr int CP = (R >= 4) ? 2 : 0;
+  constexpr int EP = (NMAX / R) * (R + CP) + 2;  // LDS doubles per line
+  static_assert(TS * EP * 8 <= 160 * 1024, "tile does not fit in LDS");
+  __shared__ __attribute__((aligned(16))) double lds[TS * EP];
+  constexpr int PER = (TS * NMAX + NT - 1) / NT; // elements per thread per tile
+  constexpr int BATCH = (PER % 8 == 0) ? 8 : 1;
+
+  const NtPlanK& P = args.plan;
+  const int n = FULL ? NMAX : P.n;
+  int tid = threadIdx.x;                         // refreshed per tile (see line_kernel_tile)
+  const int ql = threadIdx.x >> 6;               // my line in the tile
+  const int lane = threadIdx.x & 63;
+  const int my_e0 = lane * R;
+  const int64_t nsys = args.phys - 2;            // solved lines: staged lines 1 .. phys-2
+  const int64_t estride = args.estride;
+  auto at = [&](int row, int el) -> int { return row * EP + el + (el / R) * CP; };
+
+  const TileWalk walk = tile_walk(args.total, args.ostride != 0);
+  for (int64_t k = 0; k < walk.count; ++k) {
+    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);   // the mask keeps its range known
+    const int64_t t = walk.tile(k);
+    const int64_t o = t / args.tiles;
+    const int64_t p0 = (t - o * args.tiles) * TS;            // first solved line of the tile
+    const int64_t base = o * args.ostride + p0;              // staged line p0: the left neighbour of line p0
+    const double* __restrict__ in = static_cast<const double*>(args.in) + base;
+    __syncthreads();                             // the previous tile's stores have read the LDS
+
+    // ---- load and form the right-hand side ----------------------------------------
+#pragma unroll 1
+    for (int b = 0; b < PER; b += BATCH) {
+      double c[BATCH], l[BATCH], rt[BATCH];
+#pragma unroll
+      for (int j = 0; j < BATCH; ++j) {
+        const int idx = tid + (b + j) * NT;
+        const int r = idx % TS, el = idx / TS;
+        const bool ok = el < n && p0 + r < nsys;
+        const int64_t pos = (r + 1) + static_cast<int64_t>(el) * estride;   // staged line r + 1
+        c[j] = ok ? in[pos] : 0.0;
+        l[j] = ok ? in[pos - 1] : 0.0;
+        rt[j] = ok ? in[pos + 1] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < BATCH; ++j) {
+        const int idx = tid + (b + j) * NT;
+        const int r = idx % TS, el = idx / TS;
+        if (el < n)
+          lds[at(r, el)] = (el == 0 || el == n - 1) ? c[j] : args.s1 * (l[j] + rt[j]) + args.s2 * c[j];
+      }
+    }
+    __syncthreads();
+
+    // ---- solve: one line per wavefront --------------------------------------------
+    if (p0 + ql < nsys) {                        // uniform across the wavefront
+      double d[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) d[i] = (FULL || my_e0 + i < n) ? lds[at(ql, my_e0 + i)] : 0.0;
+      double x[R];
+      solve_chunk<R, W, PREC, FULL>(d, x, P, lane, args.sweeps);
+      wave_lds_sync();                           // the solution overwrites the line just read
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        if (FULL || my_e0 + i < n) lds[at(ql, my_e0 + i)] = x[i];
+    }
+    __syncthreads();
+
+    // ---- store -----------------------------------------------------------------------
+    double* __restrict__ out = static_cast<double*>(args.out) + base;
+#pragma unroll 4
+    for (int i = 0; i < PER; ++i) {
+      const int idx = tid + i * NT;
+      const int r = idx % TS, el = idx / TS;
+      if (el < n && p0 + r < nsys) out[(r + 1) + static_cast<int64_t>(el) * estride] = lds[at(r, el)];
+    }
+  }
+}
+
+
 // FP32-stored fields (compact derivative, FP64 result): a 16-line tile of FP32
 // rows reads 64-byte runs, half of what a row of FP64 gives.  This variant
 // takes 32-line tiles (128-byte FP32 runs), two lines per wavefront: the tile
