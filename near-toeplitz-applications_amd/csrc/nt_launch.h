@@ -24,13 +24,15 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //           2: workgroup tile of 4 lines staged through LDS (any n, alignment)
 //   STRIDED 0: tile of 16 adjacent lines, 64 lanes per line (1024 threads)
 //           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads);
-//              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads),
-//              for ADI above 1024 points the whole tile in LDS, RHS formed at load
+//              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads);
+//              ADI above 1024 points: whole tiles of the solved lines, RHS formed at load
 //           2: persistent whole-tile kernel, 16 lines (8 above 1024 points, and
 //              above 512 for ADI), next tile's loads in flight during this one;
 //              FP32-stored compact derivatives: 32 lines, two per wavefront
-//           3: as 2, with 32-line tiles (256-byte FP64 rows), two lines per
-//              wavefront, for lines up to 512 points outside ADI; otherwise as 2
+//           3: as 2, with two lines per wavefront, for lines up to 512 points
+//              outside ADI; otherwise as 2.  ADI lines above 1024 points (2, 3):
+//              whole tiles of the solved lines only, right-hand side formed at
+//              load (measured no faster than the halo tiles at 1024 and below)
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 constexpr int kNumVariants[2] = {3, 4};
 int tune_get(int layout, int prec, int n);
@@ -287,7 +289,7 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     }
     if (variant == 1) {
       // ADI on 1025..2048-point lines: whole 8-line tiles with the right-hand side
-      // formed at load (ADI 2048^2 step 67.8 -> 60.0 us against the blocked kernel)
+      // formed at load (ADI 2048^2 step 67.8 -> 60.0 us against the row-block kernel)
       if constexpr (PROD == P_ADI) {
         if (n > 1024 && n <= 2048) NT_GOTA(32, 8);
       }

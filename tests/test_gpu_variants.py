@@ -74,3 +74,31 @@ def test_every_strided_variant_fp32_storage(dev, precision, n, axis):
             assert err < TOL[precision], (v, err)
     finally:
         _set(1, precision, n, keep)
+
+
+@pytest.mark.parametrize("N", [34, 130, 256, 300, 512, 700, 1024, 2048])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_every_variant_adi(dev, N, layout):
+    """ADI steps under every launch shape of one layout (0: the x-implicit half
+    step on contiguous lines, 1: the y-implicit half step on strided lines, with
+    its halo lines), the other layout at its default, equal the oracle's
+    Peaceman-Rachford step."""
+    from ntamd import adi_coeffs, Plan, core
+    from ntamd.autotune import _set, current
+    from oracle import adi
+    r = 0.7
+    u0 = adi.gaussian_ic(N, seed=N)
+    want = adi.pr_run(u0, r, 1)
+    plan = Plan(N, adi_coeffs(r))
+    keep, nv = current(layout, N, "fp64")
+    try:
+        for v in range(nv):
+            _set(layout, "fp64", N, v)
+            u = torch.tensor(u0, device=dev)
+            w = torch.empty_like(u)
+            core.adi_prepare(u, w)               # the work buffer carries u's boundary values
+            core.adi_steps(plan, u, w, r, 1, "fp64", 1)
+            err = np.abs(u.cpu().numpy() - want).max() / np.abs(want).max()
+            assert err < 1e-12, (v, err)
+    finally:
+        _set(layout, "fp64", N, keep)
