@@ -16,7 +16,7 @@ with its arrival times on the wall clock:
 * on every scheduling pass the newly admitted small derivative requests go to
   the GPU together as one mixed persistent launch (``MixedBatch``); larger
   derivatives and ADI runs get their own fused launches; every launch group
-  goes to the least busy of a pool of streams, so new work overlaps work
+  goes to the stream of a pool with the fewest bytes of work in flight, so new work overlaps work
   already on the device;
 * completions are observed with events; memory is released and the latency
   (completion - arrival) recorded.
@@ -159,14 +159,16 @@ class Server:
         return (q.kind == "deriv" and math.prod(q.shape) <= self.small_points
                 and 16 <= q.shape[q.axis] <= 512)
 
-    def _stream(self):
+    def _stream(self, nbytes):
+        """The stream with the fewest bytes of work still in flight; ``nbytes`` is
+        added to it until the launch group completes."""
         i = min(range(len(self.streams)), key=lambda k: self._busy[k])
-        self._busy[i] += 1
+        self._busy[i] += nbytes
         return i
 
     # -- launching: each returns (event, stream index, [(request, result)], keep-alive) --
     def _launch_derivs(self, reqs, batched, precision=None):
-        si = self._stream()
+        si = self._stream(sum(r.nbytes() for r in reqs))
         with torch.cuda.stream(self.streams[si]):
             jobs, items = [], []
             for r in reqs:
@@ -189,7 +191,7 @@ class Server:
         return ev, si, items, keep
 
     def _launch_adi(self, r, precision=None):
-        si = self._stream()
+        si = self._stream(r.nbytes() * r.steps)
         with torch.cuda.stream(self.streams[si]):
             u = adi_field(r.side, r.seed, self.device)
             work = torch.empty_like(u)
@@ -245,9 +247,9 @@ class Server:
                     for q, res in items:
                         q.done = t
                         used -= q.nbytes()
+                        self._busy[si] -= q.nbytes() * (q.steps if q.kind == "adi" else 1)
                         if self.keep_results:
                             q.result = res
-                    self._busy[si] -= 1
                     progressed = True
                 else:
                     still.append((ev, si, items, keep))
