@@ -51,7 +51,7 @@ int g_tune[2][2][13][2] = {
     // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
     // 2 persistent whole-tile kernel, 3 the same with two lines per wavefront (two workgroups per CU)
     {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {2, 1}},
-     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 3}, {3, 3}, {3, 3}, {2, 2}, {1, 1}, {1, 1}}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 3}, {3, 3}, {3, 3}, {2, 2}, {2, 1}, {1, 1}}},
 };
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
