@@ -141,7 +141,7 @@ __device__ __forceinline__ void solve_line(double* row, int n, int prod, int pre
 __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
   __shared__ __attribute__((aligned(16))) double lds[BATCH_TS * BATCH_EP];
   __shared__ __attribute__((aligned(16))) NtPlanK s_plan;   // level table of the current job
-  __shared__ int64_t s_tile;
+  __shared__ int64_t s_tile[2];     // tile indices, alternating slots
   // reset through an opaque move at every tile, so the per-thread offsets
   // derived from it are recomputed per tile, not hoisted and held across the loop
   int tid = threadIdx.x;
@@ -149,23 +149,32 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
   const int lane = threadIdx.x & 63;
   int j = 0;                        // current job: tiles come in increasing order
   int plan_in_lds = -1;
+  int slot = 0;
 
+  // One workgroup barrier per tile, at the top: every wavefront has finished the
+  // previous tile (its stores have read the LDS, its solve is done with s_plan),
+  // and the new tile index is visible.  The index alternates between two slots,
+  // so writing the next one cannot race with late readers of this one.  Strided
+  // tiles spread each line over all threads and add two barriers; contiguous
+  // tiles are staged, solved and stored by each wavefront on its own lines.
   for (;;) {
     tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (BATCH_NT - 1);   // the mask keeps its range known
-    if (tid == 0) s_tile = static_cast<int64_t>(atomicAdd(args.counter, 1u));
+    if (tid == 0) s_tile[slot] = static_cast<int64_t>(atomicAdd(args.counter, 1u));
     __syncthreads();
     // the tile index is the same for the whole workgroup: keep it (and the
     // job index derived from it) in scalar registers
-    const int64_t t = static_cast<int64_t>(static_cast<unsigned>(uni(static_cast<int>(s_tile))));
+    const int64_t t = static_cast<int64_t>(static_cast<unsigned>(uni(static_cast<int>(s_tile[slot]))));
+    slot ^= 1;
     if (t >= args.total_tiles) break;            // every workgroup reaches this
     while (j + 1 < args.njobs && args.jobs[j + 1].tile0 <= t) ++j;
     const BatchJob& J = args.jobs[j];
     const int plan = uni(J.plan);
-    if (plan != plan_in_lds) {                   // copy the level table into LDS
+    if (plan != plan_in_lds) {                   // copy the level table into LDS (uniform branch)
       const int* src = reinterpret_cast<const int*>(args.plans + plan);
       int* dst = reinterpret_cast<int*>(&s_plan);
       for (int i = tid; i < static_cast<int>(sizeof(NtPlanK) / 4); i += BATCH_NT) dst[i] = src[i];
       plan_in_lds = plan;
+      __syncthreads();
     }
     const int64_t local = t - J.tile0;
     const int64_t o = local / J.tiles;
@@ -186,7 +195,8 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
       stage_tile(lds, static_cast<const float*>(J.in), layout, tid, rows, n, rsel, base, pstride, estride);
     else
       stage_tile(lds, static_cast<const double*>(J.in), layout, tid, rows, n, rsel, base, pstride, estride);
-    __syncthreads();
+    if (layout == CONTIG) wave_lds_sync();       // uniform: a wavefront staged its own lines
+    else __syncthreads();
 
     // ---- each wavefront solves its lines; a line's LDS row is read and then
     // overwritten by the same wavefront only, so no barrier in between.  The
@@ -214,7 +224,8 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
         }
       }
     }
-    __syncthreads();
+    if (layout == CONTIG) wave_lds_sync();
+    else __syncthreads();
 
     // ---- store ----------------------------------------------------------------------
 #pragma unroll
@@ -224,7 +235,6 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
       if (line < rows && e < n)
         __builtin_nontemporal_store(lds[line * BATCH_EP + e + (e >> rsel)], out + base + line * pstride + e * estride);
     }
-    __syncthreads();   // LDS, the plan copy and s_tile are reused by the next tile
   }
 }
 
