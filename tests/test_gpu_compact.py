@@ -179,3 +179,19 @@ def test_512cubed_properties(dev, axis):
         src = g[tuple(sl)].cpu().numpy()
         want = cr.banded_solve(compact.COMPACT_COEFFS, compact.compact_rhs(src[None], h))[0]
         assert rel_err(out[tuple(sl)].cpu().numpy(), want) < FP64_TOL
+
+
+@pytest.mark.parametrize("shape", [(512, 64, 128), (512, 33, 250)])
+def test_z_lines_far_rows(dev, shape):
+    """z-lines whose rows are 64 KiB or more apart: 8192 lines (whole aligned
+    tiles, the fast path of the whole-tile kernel) and 8250 lines (a ragged last
+    tile, the general kernel).  Both equal the oracle."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.workloads import compact_field
+    f = compact_field(shape, seed=7)
+    h = 1.0 / (shape[0] - 1)
+    want = compact.derivative_banded(f, h, 0)
+    fd = torch.tensor(f, device=dev)
+    out = torch.full_like(fd, float("nan"))
+    core.compact_derivative(Plan(shape[0], COMPACT_COEFFS), fd, out, h, 0, "fp64")
+    assert np.abs(out.cpu().numpy() - want).max() / np.abs(want).max() < 1e-12
