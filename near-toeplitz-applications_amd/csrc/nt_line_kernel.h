@@ -64,17 +64,17 @@ __device__ __forceinline__ void st_from_double(T* p, double v) {
 }
 
 // (A x)[e] of the level-0 near-Toeplitz matrix for row e of my chunk (left and
-// right: rows e-1 and e+1); the same expressions as in residual()
+// right: rows e-1 and e+1); the same expressions as in residual().  The three
+// row forms are computed from uniform (scalar) loads and the result selected:
+// with the coefficient loads inside branches on e, the compiler merges them
+// into one per-lane vector load from a selected address, and that load waits
+// behind every store still in flight (the vector-memory counter is in order),
+// so the refinement sweep could not overlap the previous stores.
 __device__ __forceinline__ double apply_row(const NtPlanK& P, int e, int n, double left, double xe, double right) {
-  double ax;
-  if (e == 0) {
-    ax = P.b1 * xe + P.c1 * right;
-  } else if (e == n - 1) {
-    ax = P.an * left + P.bn * xe;
-  } else {
-    ax = P.ai * left + P.bi * xe + P.ci * right;
-  }
-  return ax;
+  const double first = P.b1 * xe + P.c1 * right;
+  const double last = P.an * left + P.bn * xe;
+  const double inner = P.ai * left + P.bi * xe + P.ci * right;
+  return (e == 0) ? first : ((e == n - 1) ? last : inner);
 }
 
 // residual r = d - A x for the level-0 near-Toeplitz matrix, chunked layout,
@@ -91,14 +91,7 @@ __device__ __forceinline__ void residual(const double (&d)[R], const double (&x)
     const int e = lane * R + k;
     const double left = (k > 0) ? x[k - 1] : xl;
     const double right = (k + 1 < R) ? x[k + 1] : xr;
-    double ax;
-    if (e == 0) {
-      ax = P.b1 * x[k] + P.c1 * right;
-    } else if (e == n - 1) {
-      ax = P.an * left + P.bn * x[k];
-    } else {
-      ax = P.ai * left + P.bi * x[k] + P.ci * right;
-    }
+    const double ax = apply_row(P, e, n, left, x[k], right);
     r[k] = static_cast<TR>((e < n) ? d[k] - ax : 0.0);
   }
 }
