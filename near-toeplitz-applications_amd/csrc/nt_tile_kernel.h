@@ -663,4 +663,146 @@ __global__ void __launch_bounds__(1024) line_kernel_tile_f32(const LineArgs args
   }
 }
 
+
+// Fast path of line_kernel_tile_f32: n == 64 R, every 32-line tile whole and
+// 16-byte aligned (the launcher checks).  Loads and stores are unconditional,
+// with compile-time counts, and the loop is rotated with the last tile peeled,
+// as in line_kernel_tile_fast.  The next tile's values are fetched into
+// registers right after the second half's solve, when the solve's registers
+// are free, so the fetch is in flight while that half is staged and stored;
+// the deposit then waits for the fetch alone (vmcnt = the second half's store
+// count), and the stores drain behind it.
+//
+//   fetch(t0) ; deposit ; barrier ; loop { pull chunks ; barrier ; solve A ;
+//     barrier ; store A ; barrier ; solve B ; fetch(next) ; barrier ; store B ;
+//     barrier ; deposit(next) ; barrier } ; last tile without the fetch
+template <int R, int PREC>
+__global__ void __launch_bounds__(1024) line_kernel_tile_f32_fast(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
+  constexpr int W = 64;
+  constexpr int NT = 1024;
+  constexpr int HALF = 16;                       // lines solved at once, one per wavefront
+  constexpr int TS = 2 * HALF;                   // lines per tile
+  constexpr int N = W * R;                       // line length
+  constexpr int EPF = (N / R) * (R + 1) + 1;     // as line_kernel_tile_f32
+  constexpr int CP = (R >= 4) ? 2 : 0;
+  constexpr int EPD = (N / R) * (R + CP) + 2;
+  constexpr int IN_BYTES = TS * EPF * 4;
+  constexpr int OUT_BYTES = HALF * EPD * 8;
+  constexpr int LDS_BYTES = IN_BYTES > OUT_BYTES ? IN_BYTES : OUT_BYTES;
+  static_assert(LDS_BYTES <= 160 * 1024, "tile does not fit in LDS");
+  __shared__ __attribute__((aligned(16))) double smem[LDS_BYTES / 8];
+  float* ldf = reinterpret_cast<float*>(smem);
+  double* ldd = smem;
+  constexpr int VPR = TS / 4;                    // 16-byte vectors per input row
+  static_assert((VPR * N) % NT == 0, "every thread loads the same number of vectors");
+  constexpr int PFV = VPR * N / NT;              // vectors fetched per thread per tile
+  constexpr int VPO = HALF / 2;                  // 16-byte vectors per output row
+  static_assert((VPO * N) % NT == 0, "every thread stores the same number of vectors");
+  constexpr int SOV = VPO * N / NT;              // vectors stored per thread per half
+
+  const NtPlanK& P = args.plan;
+  int tid = threadIdx.x;                         // refreshed per tile (see line_kernel_tile)
+  const int ql = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int my_e0 = lane * R;
+  const int64_t estride = args.estride;
+  auto atf = [&](int row, int el) -> int { return row * EPF + el + el / R; };
+  auto atd = [&](int row, int el) -> int { return row * EPD + el + (el / R) * CP; };
+  auto base_of = [&](int64_t t) -> int64_t {
+    const int64_t o = t / args.tiles;
+    return o * args.ostride + (t - o * args.tiles) * TS;
+  };
+
+  nt_flt4 pf[PFV];
+  auto fetch = [&](int64_t base) {
+    const float* __restrict__ in = static_cast<const float*>(args.in) + base;
+#pragma unroll
+    for (int i = 0; i < PFV; ++i) {
+      const int idx = tid + i * NT;
+      const int vr = idx % VPR, el = idx / VPR;
+      pf[i] = __builtin_nontemporal_load(reinterpret_cast<const nt_flt4*>(in + el * estride + vr * 4));
+    }
+  };
+  auto deposit = [&]() {
+#pragma unroll
+    for (int i = 0; i < PFV; ++i) {
+      const int idx = tid + i * NT;
+      const int vr = idx % VPR, el = idx / VPR;
+      ldf[atf(vr * 4, el)] = pf[i].x;
+      ldf[atf(vr * 4 + 1, el)] = pf[i].y;
+      ldf[atf(vr * 4 + 2, el)] = pf[i].z;
+      ldf[atf(vr * 4 + 3, el)] = pf[i].w;
+    }
+  };
+  auto solve_stage = [&](const float (&f)[R], const float (&bnd)[6]) {
+    double x[R];
+    compact_solve_from_f32<R, W, PREC, true>(f, bnd, my_e0, N, args.s1, args.s2, x, P, lane, args.sweeps);
+#pragma unroll
+    for (int k = 0; k < R; ++k) ldd[atd(ql, my_e0 + k)] = x[k];
+  };
+  auto store_half = [&](int64_t base) {
+    double* __restrict__ dst = static_cast<double*>(args.out) + base;
+#pragma unroll
+    for (int i = 0; i < SOV; ++i) {
+      const int idx = tid + i * NT;
+      const int vr = idx % VPO, el = idx / VPO;
+      nt_dbl2 v;
+      v.x = ldd[atd(vr * 2, el)];
+      v.y = ldd[atd(vr * 2 + 1, el)];
+      __builtin_nontemporal_store(v, reinterpret_cast<nt_dbl2*>(dst + el * estride + vr * 2));
+    }
+  };
+  // Per-thread offsets are re-derived from a fresh opaque thread index after
+  // each solve, so none computed before a solve is held (spilled) across it.
+  auto refresh = [&]() { tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1); };
+  // one tile from the LDS-resident input; `next` runs after the second solve
+  auto tile = [&](int64_t base, auto&& next) {
+    float fa[R], fb[R], ba[6], bb[6];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      fa[k] = ldf[atf(ql, my_e0 + k)];
+      fb[k] = ldf[atf(ql + HALF, my_e0 + k)];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int e = (i < 3) ? i : N + 2 - i;
+      ba[i] = ldf[atf(ql, e)];
+      bb[i] = ldf[atf(ql + HALF, e)];
+    }
+    __syncthreads();                             // the input is in registers; the LDS stages results
+    solve_stage(fa, ba);
+    refresh();
+    __syncthreads();
+    store_half(base);
+    __syncthreads();                             // the first half's stores have read the LDS
+    solve_stage(fb, bb);
+    // keeps the scheduler from hoisting the fetch into the solve, where its
+    // registers would be live (and spill)
+    __builtin_amdgcn_sched_barrier(0);
+    refresh();
+    next();
+    __syncthreads();
+    store_half(base + HALF);
+  };
+
+  const TileWalk walk = tile_walk(args.total, args.ostride != 0);
+  int64_t b = base_of(walk.tile(0));
+  fetch(b);
+  deposit();
+  __syncthreads();
+  for (int64_t k = 0; k + 1 < walk.count; ++k) {
+    refresh();
+    const int64_t bn = base_of(walk.tile(k + 1));
+    tile(b, [&]() { fetch(bn); });
+    __syncthreads();                             // the second half's stores have read the LDS
+    deposit();
+    __syncthreads();
+    b = bn;
+  }
+  refresh();
+  tile(b, []() {});
+}
+
 }  // namespace nt

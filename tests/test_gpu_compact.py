@@ -81,6 +81,31 @@ def test_fused_fp32_storage(dev, precision):
         assert rel_err(out.cpu().numpy(), want) < (FP64_TOL if precision == "fp64" else REF_TOL)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("shape,axis", [
+    ((8, 1024, 128), 1),      # 1024-point y-lines, 32 whole 32-line tiles (fast path, R = 16)
+    ((1024, 8, 128), 0),      # 1024-point z-lines, rows 4 KiB apart (fast path)
+    ((512, 128, 128), 0),     # z-lines with rows 64 KiB apart: the general kernel
+    ((72, 512, 128), 1),      # 288 tiles: more tiles than workgroups, so the tile loop turns (R = 8)
+    ((40, 1024, 256), 1),     # 320 tiles at R = 16
+    ((8, 1024, 40), 1),       # 40 lines per slab: a ragged last tile, the general kernel
+])
+def test_fp32_storage_strided_tiles(dev, shape, axis, precision):
+    """FP32-stored fields along strided 512- and 1024-point lines (32-line FP32
+    tiles): the oracle differentiates the same FP32 values in FP64."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.workloads import compact_field
+    f32 = compact_field(shape, seed=9).astype(np.float32)
+    n = shape[axis]
+    h = 1.0 / (n - 1)
+    want = compact.derivative_banded(f32.astype(np.float64), h, axis)
+    fd = torch.tensor(f32, device=dev)
+    out = torch.full(shape, float("nan"), dtype=torch.float64, device=dev)
+    core.compact_derivative(Plan(n, COMPACT_COEFFS), fd, out, h, axis, precision, 1)
+    tol = FP64_TOL if precision == "fp64" else REF_TOL
+    assert rel_err(out.cpu().numpy(), want) < tol
+
+
 def test_reference_call_sequence_unfused(dev):
     """run-compact.py:57-90 verbatim on the drop-in modules: permute -> compute_rhs ->
     NearToeplitzSolver.solve -> permute, for x, y and z."""
