@@ -193,19 +193,24 @@ int64_t persistent_grid(const void* kernel, int threads);
 
 // Fast path of the FP32-row tile kernel (nt_tile_kernel.h): power-of-two lines of
 // 512 or 1024 points, whole 32-line tiles, FP32 rows and FP64 output 16-byte
-// aligned, rows less than 64 KiB apart.  Measured on 1024^3 FP32-stored fields:
-// d/dy (rows 4 KiB apart) 2.89-2.93 -> 2.51-2.64 ms with FP32 + one sweep and
-// 2.63-2.73 -> 2.50-2.60 ms with the FP64 solve; d/dz (rows 4 MiB apart) is
-// slower with it (FP64 solve 3.32 -> 3.44-3.51 ms), so z-lines keep the
-// general kernel.  0 turns it off.
+// aligned, rows less than 2 MiB apart.  Measured on MI355X (FP32 + one sweep;
+// FP64 solve of the same values):
+//   1024^3 d/dy (rows 4 KiB apart): 2.89 -> 2.48 ms; 2.68 -> 2.49 ms
+//   512^3 d/dy (2 KiB):             0.50 -> 0.43-0.47 ms; 0.43 -> 0.35 ms
+//   512^3 d/dz (1 MiB):             0.55 -> 0.50-0.51 ms; 0.47 -> 0.41 ms
+//   1024^3 d/dz (4 MiB):            3.79 -> 3.80-3.82 ms; 3.32 -> 3.43-3.44 ms
+// so rows 2 MiB or more apart keep the general kernel.  0 turns it off.
 #ifndef NT_TILE_F32_FAST
 #define NT_TILE_F32_FAST 1
+#endif
+#ifndef NT_TILE_F32_FAST_ROW_LIMIT
+#define NT_TILE_F32_FAST_ROW_LIMIT (2 << 20)
 #endif
 
 inline bool tile_f32_fast_ok(const LineArgs& a) {
   const uintptr_t i = reinterpret_cast<uintptr_t>(a.in), o = reinterpret_cast<uintptr_t>(a.out);
   return a.pstride == 1 && a.phys % 32 == 0 && i % 16 == 0 && o % 16 == 0 && a.estride % 4 == 0 &&
-         a.ostride % 4 == 0 && a.estride * static_cast<int64_t>(sizeof(float)) < 65536;
+         a.ostride % 4 == 0 && a.estride * static_cast<int64_t>(sizeof(float)) < NT_TILE_F32_FAST_ROW_LIMIT;
 }
 
 // FP32-stored compact derivative along strided lines: 32-line tiles (nt_tile_kernel.h)

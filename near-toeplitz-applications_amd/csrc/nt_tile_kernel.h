@@ -704,15 +704,18 @@ __global__ void __launch_bounds__(1024) line_kernel_tile_f32_fast(const LineArgs
 
   const NtPlanK& P = args.plan;
   int tid = threadIdx.x;                         // refreshed per tile (see line_kernel_tile)
-  const int ql = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int my_e0 = lane * R;
   const int64_t estride = args.estride;
   auto atf = [&](int row, int el) -> int { return row * EPF + el + el / R; };
   auto atd = [&](int row, int el) -> int { return row * EPD + el + (el / R) * CP; };
   auto base_of = [&](int64_t t) -> int64_t {
-    const int64_t o = t / args.tiles;
-    return o * args.ostride + (t - o * args.tiles) * TS;
+    // Opaque per call: the division's reciprocal is recomputed for each tile
+    // (a few instructions) instead of being held, and spilled, across the loop.
+    int64_t tiles = args.tiles;
+    asm volatile("" : "+s"(tiles));
+    const int64_t o = t / tiles;
+    return o * args.ostride + (t - o * tiles) * TS;
   };
 
   nt_flt4 pf[PFV];
@@ -736,11 +739,18 @@ __global__ void __launch_bounds__(1024) line_kernel_tile_f32_fast(const LineArgs
       ldf[atf(vr * 4 + 3, el)] = pf[i].w;
     }
   };
+  // The per-thread offsets are re-derived from a fresh opaque thread index
+  // after each solve, so none computed before a solve is held (spilled) across
+  // it; a scratch reload at the top of a tile would wait for the previous
+  // tile's stores (the vector-memory counter is in order).
+  auto refresh = [&]() { tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1); };
   auto solve_stage = [&](const float (&f)[R], const float (&bnd)[6]) {
     double x[R];
     compact_solve_from_f32<R, W, PREC, true>(f, bnd, my_e0, N, args.s1, args.s2, x, P, lane, args.sweeps);
+    refresh();
+    const int q = tid >> 6, e0 = (tid & 63) * R;
 #pragma unroll
-    for (int k = 0; k < R; ++k) ldd[atd(ql, my_e0 + k)] = x[k];
+    for (int k = 0; k < R; ++k) ldd[atd(q, e0 + k)] = x[k];
   };
   auto store_half = [&](int64_t base) {
     double* __restrict__ dst = static_cast<double*>(args.out) + base;
@@ -754,26 +764,23 @@ __global__ void __launch_bounds__(1024) line_kernel_tile_f32_fast(const LineArgs
       __builtin_nontemporal_store(v, reinterpret_cast<nt_dbl2*>(dst + el * estride + vr * 2));
     }
   };
-  // Per-thread offsets are re-derived from a fresh opaque thread index after
-  // each solve, so none computed before a solve is held (spilled) across it.
-  auto refresh = [&]() { tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1); };
   // one tile from the LDS-resident input; `next` runs after the second solve
   auto tile = [&](int64_t base, auto&& next) {
     float fa[R], fb[R], ba[6], bb[6];
+    const int q = tid >> 6, e0 = (tid & 63) * R;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      fa[k] = ldf[atf(ql, my_e0 + k)];
-      fb[k] = ldf[atf(ql + HALF, my_e0 + k)];
+      fa[k] = ldf[atf(q, e0 + k)];
+      fb[k] = ldf[atf(q + HALF, e0 + k)];
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       const int e = (i < 3) ? i : N + 2 - i;
-      ba[i] = ldf[atf(ql, e)];
-      bb[i] = ldf[atf(ql + HALF, e)];
+      ba[i] = ldf[atf(q, e)];
+      bb[i] = ldf[atf(q + HALF, e)];
     }
     __syncthreads();                             // the input is in registers; the LDS stages results
     solve_stage(fa, ba);
-    refresh();
     __syncthreads();
     store_half(base);
     __syncthreads();                             // the first half's stores have read the LDS

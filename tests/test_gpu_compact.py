@@ -85,7 +85,6 @@ def test_fused_fp32_storage(dev, precision):
 @pytest.mark.parametrize("shape,axis", [
     ((8, 1024, 128), 1),      # 1024-point y-lines, 32 whole 32-line tiles (fast path, R = 16)
     ((1024, 8, 128), 0),      # 1024-point z-lines, rows 4 KiB apart (fast path)
-    ((512, 128, 128), 0),     # z-lines with rows 64 KiB apart: the general kernel
     ((72, 512, 128), 1),      # 288 tiles: more tiles than workgroups, so the tile loop turns (R = 8)
     ((40, 1024, 256), 1),     # 320 tiles at R = 16
     ((8, 1024, 40), 1),       # 40 lines per slab: a ragged last tile, the general kernel
@@ -104,6 +103,30 @@ def test_fp32_storage_strided_tiles(dev, shape, axis, precision):
     core.compact_derivative(Plan(n, COMPACT_COEFFS), fd, out, h, axis, precision, 1)
     tol = FP64_TOL if precision == "fp64" else REF_TOL
     assert rel_err(out.cpu().numpy(), want) < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("shape", [(512, 32, 8192), (512, 64, 8192)])
+def test_fp32_storage_z_lines_far_rows(dev, shape, precision):
+    """FP32-stored z-lines with rows 1 MiB apart (the fast tile path) and 2 MiB
+    apart (the general one), 262,144 and 524,288 lines: 256 lines drawn at
+    random are checked against the oracle (lines are independent)."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.workloads import compact_field_device
+    n = shape[0]
+    h = 1.0 / (n - 1)
+    fd = compact_field_device(shape, seed=11, dtype=torch.float32, device=dev)
+    out = torch.full(shape, float("nan"), dtype=torch.float64, device=dev)
+    core.compact_derivative(Plan(n, COMPACT_COEFFS), fd, out, h, 0, precision, 1)
+    rng = np.random.default_rng(3)
+    ys = torch.tensor(rng.integers(0, shape[1], 256), device=dev)
+    xs = torch.tensor(rng.integers(0, shape[2], 256), device=dev)
+    cols = fd[:, ys, xs].double().cpu().numpy()[:, :, None]        # (n, 256, 1)
+    want = compact.derivative_banded(cols, h, 0)
+    got = out[:, ys, xs].cpu().numpy()[:, :, None]
+    tol = FP64_TOL if precision == "fp64" else REF_TOL
+    assert rel_err(got, want) < tol
+    del fd, out
 
 
 def test_reference_call_sequence_unfused(dev):
