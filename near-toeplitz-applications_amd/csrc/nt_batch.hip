@@ -147,6 +147,14 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
   int tid = threadIdx.x;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
+  // The job and plan tables are read through the constant address space, i.e.
+  // with scalar loads: as vector loads they would wait, in order, for the
+  // previous tile's stores before the next tile could be located and staged.
+  // The tables are written before the launch and only read by the kernel.
+  using CJob = const __attribute__((address_space(4))) BatchJob;
+  using CPlan = const __attribute__((address_space(4))) NtPlanK;
+  CJob* jobs = (CJob*)(args.jobs);
+  CPlan* plans = (CPlan*)(args.plans);
   int j = 0;                        // current job: tiles come in increasing order
   int plan_in_lds = -1;
   int slot = 0;
@@ -166,9 +174,9 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
     const int64_t t = static_cast<int64_t>(static_cast<unsigned>(uni(static_cast<int>(s_tile[slot]))));
     slot ^= 1;
     if (t >= args.total_tiles) break;            // every workgroup reaches this
-    while (j + 1 < args.njobs && args.jobs[j + 1].tile0 <= t) ++j;
-    const BatchJob& J = args.jobs[j];
-    const int plan = uni(J.plan);
+    while (j + 1 < args.njobs && jobs[j + 1].tile0 <= t) ++j;
+    const CJob& J = jobs[j];
+    const int plan = J.plan;
     if (plan != plan_in_lds) {                   // copy the level table into LDS (uniform branch)
       const int* src = reinterpret_cast<const int*>(args.plans + plan);
       int* dst = reinterpret_cast<int*>(&s_plan);
@@ -182,8 +190,8 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
     const int64_t base = o * J.ostride + p0 * J.pstride;
     const int64_t rows64 = J.phys - p0;
     const int rows = rows64 < BATCH_TS ? static_cast<int>(rows64) : BATCH_TS;
-    const int layout = uni(J.layout), prod = uni(J.prod), prec = uni(J.prec);
-    const int n = uni(args.plans[plan].n);
+    const int layout = J.layout, prod = J.prod, prec = J.prec;
+    const int n = plans[plan].n;
     const int64_t pstride = J.pstride, estride = J.estride;
     double* __restrict__ out = static_cast<double*>(J.out);
 
@@ -191,7 +199,7 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
     const int rsel = (n <= 64) ? 0 : (n <= 128) ? 1 : (n <= 256) ? 2 : 3;
 
     // ---- stage the tile (TS lines x n) through LDS, coalesced along memory ------
-    if (uni(J.in_f32))
+    if (J.in_f32)
       stage_tile(lds, static_cast<const float*>(J.in), layout, tid, rows, n, rsel, base, pstride, estride);
     else
       stage_tile(lds, static_cast<const double*>(J.in), layout, tid, rows, n, rsel, base, pstride, estride);
@@ -203,6 +211,11 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
     // compile-time level structure applies when n == 64 R. -----------------------
     const NtPlanK& P = s_plan;
     const bool full = (n == (64 << rsel));
+    // the job's scalars in scalar registers: loaded inside the solve they would
+    // be vector loads, and each would wait for the previous tile's stores (the
+    // vector-memory counter is in order)
+    const double s1 = J.s1, s2 = J.s2;
+    const int sweeps = J.sweeps;
 #pragma unroll 1
     for (int h = 0; h < BATCH_LPW; ++h) {
       const int ql = wave + h * BATCH_WAVES;
@@ -210,17 +223,17 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
       double* row = lds + ql * BATCH_EP;
       if (full) {
         switch (rsel) {
-          case 0: solve_line<1, true>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
-          case 1: solve_line<2, true>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
-          case 2: solve_line<4, true>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
-          default: solve_line<8, true>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+          case 0: solve_line<1, true>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
+          case 1: solve_line<2, true>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
+          case 2: solve_line<4, true>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
+          default: solve_line<8, true>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
         }
       } else {
         switch (rsel) {
-          case 0: solve_line<1, false>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
-          case 1: solve_line<2, false>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
-          case 2: solve_line<4, false>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
-          default: solve_line<8, false>(row, n, prod, prec, P, J.s1, J.s2, J.sweeps, lane); break;
+          case 0: solve_line<1, false>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
+          case 1: solve_line<2, false>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
+          case 2: solve_line<4, false>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
+          default: solve_line<8, false>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
         }
       }
     }
