@@ -46,7 +46,8 @@ int nt_version(void);
  * block per system, neato.py:181-183) --------------------------------------
  * Launch-shape variant used for lines of length n (bucketed by ceil(log2 n)
  * and by whether n is a power of two) in layout 0 (contiguous lines) or 1
- * (strided lines) at the given precision mode (NT_PREC_*).  Process-wide; the defaults are the shapes measured
+ * (strided lines) of the solves and derivatives, or 2 / 3 (the same for the
+ * ADI half steps, nt_adi_steps) at the given precision mode (NT_PREC_*).  Process-wide; the defaults are the shapes measured
  * fastest on MI355X, and the autotuner (ntamd/autotune.py) can re-measure and
  * set them from its JSON cache.  Every variant gives the same results to
  * within one rounding; only the launch shape changes. */

@@ -44,12 +44,21 @@ namespace {
 // the 512- and 1024-point strided slots taken from whole 512^3 / 1024^3 fields
 // and the 2048-point strided slot from the ADI y half step (whose lines do not
 // fit the whole-tile kernel).
-int g_tune[2][2][13][2] = {
+int g_tune[4][2][13][2] = {
     // contiguous lines: 0 wave-private transpose, 1 direct chunk loads, 2 workgroup tile
     {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 1}, {1, 1}},
      {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 1}, {0, 1}}},
     // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
     // 2 persistent whole-tile kernel, 3 the same with two lines per wavefront (two workgroups per CU)
+    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {2, 1}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 3}, {3, 3}, {3, 3}, {2, 2}, {2, 1}, {1, 1}}},
+    // ADI x half step (contiguous lines), variants as layout 0: at 2048 points the
+    // wave-private transpose, which forms the stencil from coalesced reads of the
+    // neighbouring lines (ADI 2048^2 step 58.7 -> 52.3 us; the compact derivative
+    // at 2048 points stays faster with direct chunk loads, 37.4 against 46.5 us)
+    {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 0}, {1, 1}},
+     {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 1}}},
+    // ADI y half step (strided lines, with their halo lines), variants as layout 1
     {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {2, 1}},
      {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 3}, {3, 3}, {3, 3}, {2, 2}, {2, 1}, {1, 1}}},
 };
@@ -83,10 +92,10 @@ int64_t nt::persistent_grid(const void* kernel, int threads) {
 }
 
 int nt::tune_get(int layout, int prec, int n) {
-  return g_tune[layout & 1][prec & 1][log2_bucket(n)][is_pow2(n) ? 1 : 0];
+  return g_tune[layout & 3][prec & 1][log2_bucket(n)][is_pow2(n) ? 1 : 0];
 }
 void nt::tune_set(int layout, int prec, int n, int variant) {
-  g_tune[layout & 1][prec & 1][log2_bucket(n)][is_pow2(n) ? 1 : 0] = variant;
+  g_tune[layout & 3][prec & 1][log2_bucket(n)][is_pow2(n) ? 1 : 0] = variant;
 }
 
 namespace {
@@ -160,10 +169,10 @@ extern "C" {
 
 const char* nt_last_error(void) { return g_err; }
 
-int nt_version(void) { return 101; }
+int nt_version(void) { return 102; }
 
 int nt_tune_set(int layout, int precision, int64_t n, int variant) {
-  if (layout != 0 && layout != 1) return fail(NT_EINVAL, "nt_tune_set: layout must be 0 or 1");
+  if (layout < 0 || layout > 3) return fail(NT_EINVAL, "nt_tune_set: layout must be 0..3");
   if (precision != NT_PREC_F64 && precision != NT_PREC_F32_REFINE)
     return fail(NT_EINVAL, "nt_tune_set: precision %d", precision);
   if (n < 2 || n > NT_MAX_N) return fail(NT_EINVAL, "nt_tune_set: n out of range");
@@ -175,7 +184,7 @@ int nt_tune_set(int layout, int precision, int64_t n, int variant) {
 }
 
 int nt_tune_get(int layout, int precision, int64_t n, int* variant, int* nvariants) {
-  if (layout != 0 && layout != 1) return fail(NT_EINVAL, "nt_tune_get: layout must be 0 or 1");
+  if (layout < 0 || layout > 3) return fail(NT_EINVAL, "nt_tune_get: layout must be 0..3");
   if (precision != NT_PREC_F64 && precision != NT_PREC_F32_REFINE)
     return fail(NT_EINVAL, "nt_tune_get: precision %d", precision);
   if (n < 2 || n > NT_MAX_N) return fail(NT_EINVAL, "nt_tune_get: n out of range");

@@ -34,7 +34,10 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //              whole tiles of the solved lines only, right-hand side formed at
 //              load (measured no faster than the halo tiles at 1024 and below)
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
-constexpr int kNumVariants[2] = {3, 4};
+// launch-shape variants per tuning layout: 0 contiguous, 1 strided, 2 and 3 the
+// same for the ADI half steps (their right-hand side reads neighbouring lines,
+// which favours other shapes than the derivative's)
+constexpr int kNumVariants[4] = {3, 4, 3, 4};
 int tune_get(int layout, int prec, int n);
 void tune_set(int layout, int prec, int n, int variant);
 
@@ -254,7 +257,7 @@ inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
 template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
 hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
   const int n = a.plan.n;
-  const int variant = tune_get(LAYOUT, PREC, n);
+  const int variant = tune_get(LAYOUT + (PROD == P_ADI ? 2 : 0), PREC, n);
   if constexpr (LAYOUT == CONTIG) {
     const int vin = 16 / static_cast<int>(sizeof(TIn)), vout = 16 / static_cast<int>(sizeof(TOut));
     const bool aligned = n > 64 && direct_ok(a, sizeof(TIn), sizeof(TOut));

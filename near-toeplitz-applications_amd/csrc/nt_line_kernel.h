@@ -497,7 +497,23 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
   const int nv4 = FULL ? W * R / 4 : n / 4;
   for (int s = 0; s < nl; ++s) {
     const TIn* __restrict__ src = static_cast<const TIn*>(args.in) + (q0 + s + HALO) * args.pstride;
-    if constexpr (sizeof(TIn) == 8) {
+    if constexpr (sizeof(TIn) == 8 && PROD == P_ADI) {
+      // The explicit stencil across lines is formed while streaming: the two
+      // neighbouring lines are read with the same coalesced 16-byte accesses
+      // (per-lane chunk reads of them touch 64 cache lines per instruction, and
+      // the 2048-point ADI half step waited on that address traffic).  Plain
+      // loads: each line is read again as a neighbour by two other wavefronts.
+      const nt_dbl2* v = reinterpret_cast<const nt_dbl2*>(src);
+      const nt_dbl2* vu = reinterpret_cast<const nt_dbl2*>(src - args.pstride);
+      const nt_dbl2* vd = reinterpret_cast<const nt_dbl2*>(src + args.pstride);
+#pragma unroll
+      for (int j = l64; j < nv2; j += 64) {
+        const nt_dbl2 c = v[j], u = vu[j], w = vd[j];
+        const int e = 2 * j;
+        L[at(s, e)] = (e == 0 || e >= n - 1) ? c.x : args.s1 * (u.x + w.x) + args.s2 * c.x;
+        L[at(s, e + 1)] = (e + 1 >= n - 1) ? c.y : args.s1 * (u.y + w.y) + args.s2 * c.y;
+      }
+    } else if constexpr (sizeof(TIn) == 8) {
       const nt_dbl2* v = reinterpret_cast<const nt_dbl2*>(src);
 #pragma unroll
       for (int j = l64; j < nv2; j += 64) {
@@ -542,7 +558,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
     const double fl = shup<W>(f[R - 1], 1);
     const double fr = shdn<W>(f[0], 1);
     compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
-  } else if (PROD == P_ADI) {
+  } else if (PROD == P_ADI && sizeof(TIn) != 8) {
     const TIn* __restrict__ line =
         static_cast<const TIn*>(args.in) + (q0 + seg + HALO) * args.pstride;
     double up[R], dn[R];
@@ -553,7 +569,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
       const int e = my_e0 + k;
       d[k] = (e == 0 || e >= n - 1) ? f[k] : args.s1 * (up[k] + dn[k]) + args.s2 * f[k];
     }
-  } else {
+  } else {                                    // solve as staged (FP64 ADI: the RHS already)
 #pragma unroll
     for (int k = 0; k < R; ++k) d[k] = f[k];
   }

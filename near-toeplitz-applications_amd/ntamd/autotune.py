@@ -4,7 +4,9 @@ For each layout, precision mode and line-length bucket (powers of two in a
 slot of their own) the library has several equivalent launch shapes
 (``nt_tune_set``): contiguous lines can stream through a wave-private
 LDS transpose, load per-lane chunks directly, or go through a workgroup tile;
-strided lines can use tiles of 16 or of 32 adjacent lines.  ``tune()`` times
+strided lines can use tiles of 16 or of 32 adjacent lines, or the persistent
+whole-tile kernels.  Layouts 0 and 1 serve the solves and derivatives, 2 and 3
+the ADI half steps (x-implicit on contiguous lines, y-implicit on strided ones).  ``tune()`` times
 every variant on a synthetic field of the given shape class, keeps the fastest,
 and ``save()``/``apply()`` persist the choice in a JSON cache keyed by device
 name and library version.  Variants give the same results (the strided ones to
@@ -17,7 +19,7 @@ import os
 import torch
 
 from . import _lib
-from .core import Plan, compact_derivative, precision_code
+from .core import Plan, adi_prepare, adi_steps, compact_derivative, precision_code
 
 __all__ = ["tune", "apply", "save", "load", "current", "DEFAULT_CACHE"]
 
@@ -57,13 +59,14 @@ def _time(fn, reps):
 
 
 def tune(ns=(64, 128, 256, 512, 1024, 2048), points=1 << 27, reps=10, precisions=("fp64", "fp32_refine"),
-         verbose=False):
+         verbose=False, adi=True):
     """Time every variant for contiguous (layout 0) and strided (layout 1) lines
     at each precision mode, on fields of about ``points`` points, and apply the
-    fastest.  The table has two slots per length bucket: powers of two (timed at
+    fastest; with ``adi``, also the ADI half steps (layouts 2 and 3) on n x n
+    fields.  The table has two slots per length bucket: powers of two (timed at
     n, where the levels are known at compile time) and other lengths (timed at
     3n/4, the general path).  Returns
-    ``{"<layout>:<precision>:<n>": {"variant": v, "seconds": [...]}}`` for both."""
+    ``{"<layout>:<precision>:<n>": {"variant": v, "seconds": [...]}}``."""
     dev = torch.device("cuda")
     result = {}
     for n in ns:
@@ -89,6 +92,30 @@ def tune(ns=(64, 128, 256, 512, 1024, 2048), points=1 << 27, reps=10, precisions
                         print(f"layout {layout} {prec} n {length}: " +
                               ", ".join(f"v{i} {t * 1e6:.1f}us" for i, t in enumerate(times)), flush=True)
                 del f, out
+    if adi:
+        from . import adi_coeffs
+        for n in ns:
+            for length in ([n] if n < 16 else [n, 3 * n // 4]):
+                if length < 8:
+                    continue
+                plan = Plan(length, adi_coeffs(0.5))
+                u = torch.randn((length, length), dtype=torch.float64, device=dev)
+                w = torch.empty_like(u)
+                adi_prepare(u, w)
+                for layout in (2, 3):
+                    for prec in precisions:
+                        _, nv = current(layout, length, prec)
+                        times = []
+                        for v in range(nv):
+                            _set(layout, prec, length, v)
+                            times.append(_time(lambda: adi_steps(plan, u, w, 0.5, 4, prec), reps) / 4)
+                        best = min(range(nv), key=lambda i: times[i])
+                        _set(layout, prec, length, best)
+                        result[f"{layout}:{prec}:{length}"] = {"variant": best, "seconds": times}
+                        if verbose:
+                            print(f"layout {layout} (ADI) {prec} n {length}: " +
+                                  ", ".join(f"v{i} {t * 1e6:.1f}us" for i, t in enumerate(times)), flush=True)
+                del u, w
     return result
 
 

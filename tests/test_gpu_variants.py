@@ -77,12 +77,12 @@ def test_every_strided_variant_fp32_storage(dev, precision, n, axis):
 
 
 @pytest.mark.parametrize("N", [34, 130, 256, 300, 512, 700, 1024, 2048])
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [2, 3])
 def test_every_variant_adi(dev, N, layout):
-    """ADI steps under every launch shape of one layout (0: the x-implicit half
-    step on contiguous lines, 1: the y-implicit half step on strided lines, with
-    its halo lines), the other layout at its default, equal the oracle's
-    Peaceman-Rachford step."""
+    """ADI steps under every launch shape of one ADI tuning layout (2: the
+    x-implicit half step on contiguous lines, 3: the y-implicit half step on
+    strided lines, with its halo lines), the other layout at its default, equal
+    the oracle's Peaceman-Rachford step."""
     from ntamd import adi_coeffs, Plan, core
     from ntamd.autotune import _set, current
     from oracle import adi
