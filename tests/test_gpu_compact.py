@@ -129,6 +129,31 @@ def test_fp32_storage_z_lines_far_rows(dev, shape, precision):
     del fd, out
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("shape", [(1024, 1024, 768), (1024, 999, 801)])
+def test_z_lines_rows_6mib_apart(dev, shape, precision):
+    """FP64 z-lines with rows at least 6 MiB apart, which the general tile kernel
+    walks in contiguous ranges: 786,432 lines in whole tiles and 800,199 lines
+    with a ragged last tile.  256 lines drawn at random equal the oracle."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.workloads import compact_field_device
+    n = shape[0]
+    h = 1.0 / (n - 1)
+    fd = compact_field_device(shape, seed=13, device=dev)
+    out = torch.full(shape, float("nan"), dtype=torch.float64, device=dev)
+    core.compact_derivative(Plan(n, COMPACT_COEFFS), fd, out, h, 0, precision, 1)
+    rng = np.random.default_rng(5)
+    ys = torch.tensor(rng.integers(0, shape[1], 256), device=dev)
+    xs = torch.tensor(rng.integers(0, shape[2], 256), device=dev)
+    cols = fd[:, ys, xs].cpu().numpy()[:, :, None]
+    want = compact.derivative_banded(cols, h, 0)
+    got = out[:, ys, xs].cpu().numpy()[:, :, None]
+    tol = FP64_TOL if precision == "fp64" else REF_TOL
+    assert rel_err(got, want) < tol
+    assert not torch.isnan(out[:, -1, -1]).any()     # the last (ragged) tile's last line
+    del fd, out
+
+
 def test_reference_call_sequence_unfused(dev):
     """run-compact.py:57-90 verbatim on the drop-in modules: permute -> compute_rhs ->
     NearToeplitzSolver.solve -> permute, for x, y and z."""
