@@ -177,9 +177,8 @@ def config4_case(N=1024, reps=5):
     import torch
     from ntamd import COMPACT_COEFFS, Plan, core
     from ntamd.workloads import compact_field_device
-    # Blocks cached from the earlier cases go back to the driver first: 8 GiB
-    # fields allocated beside them have run z-lines (rows 8 MiB apart, TLB-bound)
-    # at 6.6-6.8 ms instead of 4.4 ms.
+    # Blocks cached from the earlier cases go back to the driver first, so the
+    # 1024^3 fields are fresh allocations.
     torch.cuda.empty_cache()
     f32 = compact_field_device(N, seed=0, dtype=torch.float32)
     f64 = f32.double()
