@@ -681,10 +681,10 @@ __global__ void __launch_bounds__(1024) line_kernel_tile_f32(const LineArgs args
 // 16-byte aligned (the launcher checks).  Loads and stores are unconditional,
 // with compile-time counts, and the loop is rotated with the last tile peeled,
 // as in line_kernel_tile_fast.  The next tile's values are fetched into
-// registers right after the second half's solve, when the solve's registers
-// are free, so the fetch is in flight while that half is staged and stored;
-// the deposit then waits for the fetch alone (vmcnt = the second half's store
-// count), and the stores drain behind it.
+// registers right after the second half is solved and staged, when the
+// solve's registers are free, so the fetch is in flight while that half is
+// stored; the deposit then waits for the fetch alone (vmcnt = the second
+// half's store count), and the stores drain behind it.
 //
 //   fetch(t0) ; deposit ; barrier ; loop { pull chunks ; barrier ; solve A ;
 //     barrier ; store A ; barrier ; solve B ; fetch(next) ; barrier ; store B ;
@@ -801,7 +801,6 @@ __global__ void __launch_bounds__(1024) line_kernel_tile_f32_fast(const LineArgs
     // keeps the scheduler from hoisting the fetch into the solve, where its
     // registers would be live (and spill)
     __builtin_amdgcn_sched_barrier(0);
-    refresh();
     next();
     __syncthreads();
     store_half(base + HALF);
