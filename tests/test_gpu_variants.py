@@ -102,3 +102,25 @@ def test_every_variant_adi(dev, N, layout):
             assert err < 1e-12, (v, err)
     finally:
         _set(layout, "fp64", N, keep)
+
+
+def test_autotune_times_all_four_layouts(dev):
+    """ntamd.autotune.tune() on one small length: it times every variant of the
+    four tuning layouts (derivatives on layouts 0/1, ADI steps on 2/3), applies
+    the fastest, and reports each choice; the table is put back afterwards so
+    the tests after this one see the defaults."""
+    from ntamd import autotune
+    lengths = (64, 48)
+    keep = {(lay, p, n): autotune.current(lay, n, p)[0]
+            for lay in range(4) for p in ("fp64", "fp32_refine") for n in lengths}
+    try:
+        res = autotune.tune(ns=(64,), points=1 << 16, reps=1)
+        for (lay, p, n) in keep:
+            entry = res[f"{lay}:{p}:{n}"]
+            nv = autotune.current(lay, n, p)[1]
+            assert len(entry["seconds"]) == nv and 0 <= entry["variant"] < nv
+            assert autotune.current(lay, n, p)[0] == entry["variant"]
+            assert all(t > 0 for t in entry["seconds"])
+    finally:
+        for (lay, p, n), v in keep.items():
+            autotune._set(lay, p, n, v)
