@@ -172,10 +172,36 @@ int64_t persistent_grid(const void* kernel, int threads);
     return hipGetLastError();                                                              \
   } while (0)
 
+// 16-byte row accesses in the ADI tile kernel (line_kernel_tile_adi_vec): lines of
+// exactly 64 R points, aligned 8-line tiles covering the staged lines, 16-byte
+// aligned rows.  0 turns it off.
+#ifndef NT_TILE_ADI_VEC
+#define NT_TILE_ADI_VEC 1
+#endif
+
+inline bool tile_adi_vec_ok(const LineArgs& a) {
+  const uintptr_t i = reinterpret_cast<uintptr_t>(a.in), o = reinterpret_cast<uintptr_t>(a.out);
+  return a.pstride == 1 && a.phys % 8 == 0 && i % 16 == 0 && o % 16 == 0 && a.estride % 2 == 0 &&
+         a.ostride % 2 == 0;
+}
+
 // ADI along strided lines with the right-hand side formed at load (nt_tile_kernel.h)
 #define NT_GOTA(R_, WAVES_)                                                                \
   do {                                                                                     \
     if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    if constexpr (NT_TILE_ADI_VEC && (WAVES_) == 8) {                                      \
+      if (is_full(n, 64, R_) && tile_adi_vec_ok(a)) {                                      \
+        a.tiles = a.phys / 8;                                                              \
+        a.total = outer * a.tiles;                                                         \
+        if (a.total <= 0) return hipSuccess;                                               \
+        const void* kv_ = reinterpret_cast<const void*>(&line_kernel_tile_adi_vec<R_, PREC>); \
+        const int64_t capv_ = persistent_grid(kv_, 512);                                   \
+        const int64_t gridv_ = a.total < capv_ ? a.total : capv_;                          \
+        hipLaunchKernelGGL((line_kernel_tile_adi_vec<R_, PREC>), dim3(static_cast<unsigned>(gridv_)), \
+                           dim3(512), 0, st, a);                                           \
+        return hipGetLastError();                                                          \
+      }                                                                                    \
+    }                                                                                      \
     const int64_t nsys_ = a.phys - 2;                                                      \
     a.tiles = (nsys_ + (WAVES_)-1) / (WAVES_);                                             \
     a.total = outer * a.tiles;                                                             \

@@ -527,6 +527,116 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_adi(const LineArg
 }
 
 
+// line_kernel_tile_adi with 16-byte row accesses: tiles are the aligned staged
+// lines 8t .. 8t+7 (the launcher checks n == 64 R, a whole number of tiles and
+// 16-byte aligned rows), so each row of a tile is four aligned pairs of lines,
+// one pair per lane, and the four lanes of a row form a DPP quad.  A lane's
+// left and right neighbours come from the adjacent lanes of its quad (quad
+// permutes, no LDS traffic); the two lanes at the ends of a row read the one
+// line beyond the tile on either side.  The boundary lines 0 and phys-1 are not
+// solved: their staged values are written back unchanged, the value the output
+// holds there (the work buffer carries the boundary).  Per element this is
+// half a 16-byte load plus a quarter of an 8-byte halo load, against three
+// 8-byte loads in line_kernel_tile_adi, and half a 16-byte store against one
+// 8-byte store.
+template <int R, int PREC>
+__global__ void __launch_bounds__(512) line_kernel_tile_adi_vec(const LineArgs args_by_value) {
+  (void)args_by_value;
+  const LineArgs& args = kernel_args();
+  constexpr int W = 64;
+  constexpr int WAVES = 8;
+  constexpr int NT = WAVES * 64;
+  constexpr int TS = WAVES;                      // lines per tile, one per wavefront
+  constexpr int N = W * R;                       // line length
+  constexpr int CP = (R >= 4) ? 2 : 0;
+  constexpr int EP = (N / R) * (R + CP) + 2;     // as line_kernel_tile_adi
+  static_assert(TS * EP * 8 <= 160 * 1024, "tile does not fit in LDS");
+  __shared__ __attribute__((aligned(16))) double lds[TS * EP];
+  constexpr int PV = (TS / 2) * N / NT;          // line pairs per thread per tile
+  static_assert((TS / 2) * N % NT == 0, "every thread moves the same number of pairs");
+  constexpr int QUAD_NEXT = 0xF9;                // quad_perm(1, 2, 3, 3): value of lane + 1
+  constexpr int QUAD_PREV = 0x90;                // quad_perm(0, 0, 1, 2): value of lane - 1
+
+  const NtPlanK& P = args.plan;
+  int tid = threadIdx.x;                         // refreshed per tile (see line_kernel_tile)
+  const int ql = threadIdx.x >> 6;               // my line in the tile
+  const int lane = threadIdx.x & 63;
+  const int my_e0 = lane * R;
+  const int64_t phys = args.phys;
+  const int64_t estride = args.estride;
+  auto at = [&](int row, int el) -> int { return row * EP + el + (el / R) * CP; };
+
+  const TileWalk walk = tile_walk(args.total, args.ostride != 0);
+  for (int64_t k = 0; k < walk.count; ++k) {
+    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);   // the mask keeps its range known
+    const int64_t t = walk.tile(k);
+    const int64_t o = t / args.tiles;
+    const int64_t x0 = (t - o * args.tiles) * TS;            // first staged line of the tile
+    const int64_t base = o * args.ostride + x0;
+    const double* __restrict__ in = static_cast<const double*>(args.in) + base;
+    __syncthreads();                             // the previous tile's stores have read the LDS
+
+    // ---- load and form the right-hand side ----------------------------------------
+    const int pr = tid & 3;                      // my pair of lines: 2 pr, 2 pr + 1
+    // the halo line an end lane reads, clamped into the field (the value is
+    // then used only by a boundary line, which is not solved)
+    int64_t hx = (pr == 0) ? x0 - 1 : x0 + TS;
+    hx = hx < 0 ? 0 : (hx > phys - 1 ? phys - 1 : hx);
+    const int64_t hoff = hx - x0;
+    const bool bnd0 = (x0 + 2 * pr == 0), bnd1 = (x0 + 2 * pr + 1 == phys - 1);
+    nt_dbl2 v[PV];
+    double h[PV];
+#pragma unroll
+    for (int i = 0; i < PV; ++i) {
+      const int el = (tid >> 2) + i * (NT / 4);
+      const double* row = in + static_cast<int64_t>(el) * estride;
+      v[i] = *reinterpret_cast<const nt_dbl2*>(row + 2 * pr);
+      h[i] = row[hoff];
+    }
+#pragma unroll
+    for (int i = 0; i < PV; ++i) {
+      const int el = (tid >> 2) + i * (NT / 4);
+      // The quad moves run on every lane: the empty asm uses their results
+      // unconditionally, so the compiler cannot sink them into the lane-masked
+      // side of the selects below, where a lane reading a masked-off
+      // neighbour would get 0 instead of its value.
+      double prev = dpp<QUAD_PREV>(v[i].y), next = dpp<QUAD_NEXT>(v[i].x);
+      asm volatile("" : "+v"(prev), "+v"(next));
+      const double left = (pr == 0) ? h[i] : prev;     // line 2 pr - 1
+      const double right = (pr == 3) ? h[i] : next;    // line 2 pr + 2
+      const bool ident = (el == 0 || el == N - 1);
+      lds[at(2 * pr, el)] = (ident || bnd0) ? v[i].x : args.s1 * (left + v[i].y) + args.s2 * v[i].x;
+      lds[at(2 * pr + 1, el)] = (ident || bnd1) ? v[i].y : args.s1 * (v[i].x + right) + args.s2 * v[i].y;
+    }
+    __syncthreads();
+
+    // ---- solve: one line per wavefront; the boundary lines stay as staged -----------
+    const int64_t xl = x0 + ql;
+    if (xl >= 1 && xl <= phys - 2) {             // uniform across the wavefront
+      double d[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) d[i] = lds[at(ql, my_e0 + i)];
+      double x[R];
+      solve_chunk<R, W, PREC, true>(d, x, P, lane, args.sweeps);
+      wave_lds_sync();                           // the solution overwrites the line just read
+#pragma unroll
+      for (int i = 0; i < R; ++i) lds[at(ql, my_e0 + i)] = x[i];
+    }
+    __syncthreads();
+
+    // ---- store -----------------------------------------------------------------------
+    double* __restrict__ out = static_cast<double*>(args.out) + base;
+#pragma unroll
+    for (int i = 0; i < PV; ++i) {
+      const int el = (tid >> 2) + i * (NT / 4);
+      nt_dbl2 w;
+      w.x = lds[at(2 * pr, el)];
+      w.y = lds[at(2 * pr + 1, el)];
+      __builtin_nontemporal_store(w, reinterpret_cast<nt_dbl2*>(out + static_cast<int64_t>(el) * estride + 2 * pr));
+    }
+  }
+}
+
 // FP32-stored fields (compact derivative, FP64 result): a 16-line tile of FP32
 // rows reads 64-byte runs, half of what a row of FP64 gives.  This variant
 // takes 32-line tiles (128-byte FP32 runs), two lines per wavefront: the tile
