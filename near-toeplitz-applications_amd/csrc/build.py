@@ -36,7 +36,8 @@ SOURCES = [
 ARCH = os.environ.get("NT_OFFLOAD_ARCH", "gfx950")
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
-    # packed FP32 instructions stay off (see the build notes in DESIGN.md)
+    # packed FP32 instructions (v_pk_*_f32) stay off: some of their forms have
+    # given wrong results on these GPUs without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]

@@ -1,7 +1,8 @@
 """GPU parity of the near-Toeplitz solves against the CPU oracle.
 
 Tolerances (relative, ||x - x_ref||_inf / ||x_ref||_inf): 1e-12 for the FP64
-solve, 1e-10 for FP32 + one FP64-residual refinement sweep.
+solve, 1e-10 for FP32 + one FP64-residual refinement sweep (the accuracy each
+mode is specified to reach).
 """
 import numpy as np
 import pytest
