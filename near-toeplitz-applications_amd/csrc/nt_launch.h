@@ -118,12 +118,26 @@ inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) ==
 #ifndef NT_TILE_FAST
 #define NT_TILE_FAST 1
 #endif
-// Largest rows per lane of the FP64 fast path.  At 1024 points (R = 16, the
-// prefetch split around the solve, no spills) it was measured on 1024^3 d/dz
-// with five placements of the output: 3.9-4.0 ms at three, 5.6-6.0 ms at the
-// other two, where the general kernel takes 4.3-4.5 ms at all five.  So 8.
+// Largest rows per lane of the FP64 fast path.  At 1024 points (R = 16) it is
+// taken for rows less than NT_TILE_FAST_R16_ROW_LIMIT apart only: on 1024^3
+// d/dz (rows 8 MiB apart) it ran 3.9-4.0 ms at three of five output placements
+// and 5.6-6.0 ms at the other two, where the general kernel with contiguous tile
+// ranges stays at 4.2-4.7 ms.  1024^3 d/dy: 3.47 -> 3.40 ms.
 #ifndef NT_TILE_FAST_MAX_R
-#define NT_TILE_FAST_MAX_R 8
+#define NT_TILE_FAST_MAX_R 16
+#endif
+
+// At 16 rows per lane the fast path takes only lines whose rows are closer than
+// this many bytes (y-lines); z-lines keep the general kernel and its tile order.
+#ifndef NT_TILE_FAST_R16_ROW_LIMIT
+#define NT_TILE_FAST_R16_ROW_LIMIT (2 << 20)
+#endif
+// Largest rows per lane of the FP32 + refinement fast path (one line per
+// wavefront).  At 16 rows per lane the next tile is fetched after the solve and
+// the solve's LDS offsets are re-derived after it, which keeps the loop free of
+// spills: FP64-stored 1024^3 d/dy with FP32 + one sweep 5.61 -> 3.45 ms.
+#ifndef NT_TILE_FAST_F32R_MAX_R
+#define NT_TILE_FAST_F32R_MAX_R 16
 #endif
 
 // Whole 16-byte aligned tiles of `ts` adjacent strided lines only: the fast
@@ -148,8 +162,9 @@ int64_t persistent_grid(const void* kernel, int threads);
     const bool full_ = is_full(n, 64, R_);                                                 \
     if constexpr (NT_TILE_FAST && std::is_same_v<TIn, double> && std::is_same_v<TOut, double> &&     \
                   PROD != P_ADI && ((R_) * (LPW_)) % 2 == 0 &&                             \
-                  (R_) <= (PREC == PREC_F64 ? NT_TILE_FAST_MAX_R : ((LPW_) == 1 ? 8 : 4))) { \
-      if (full_ && tile_fast_ok(a, (WAVES_) * (LPW_))) {                                   \
+                  (R_) <= (PREC == PREC_F64 ? NT_TILE_FAST_MAX_R : ((LPW_) == 1 ? NT_TILE_FAST_F32R_MAX_R : 4))) { \
+      if (full_ && tile_fast_ok(a, (WAVES_) * (LPW_)) &&                                   \
+          ((R_) < 16 || a.estride * static_cast<int64_t>(sizeof(TIn)) < NT_TILE_FAST_R16_ROW_LIMIT)) { \
         const void* kf_ = reinterpret_cast<const void*>(&line_kernel_tile_fast<R_, WAVES_, LPW_, PROD, PREC>); \
         const int64_t capf_ = persistent_grid(kf_, (WAVES_)*64);                           \
         const int64_t gridf_ = a.total < capf_ ? a.total : capf_;                          \

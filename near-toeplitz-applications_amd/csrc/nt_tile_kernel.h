@@ -331,7 +331,13 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
   // Long lines (R = 16) fetch the first half of the next tile before the solve
   // and the second half after it, still ahead of this tile's stores: half the
   // prefetch registers are live during the solve.
-  constexpr int PF_EARLY = (R >= 16) ? PFV / 2 : PFV;
+  // FP32 + refinement at 16 rows per lane fetches it all after the solve: its
+  // solve alone fills the register budget.
+  constexpr int PF_EARLY = (R >= 16) ? ((PREC == PREC_F32R) ? 0 : PFV / 2) : PFV;
+  // At 16 rows per lane the LDS offsets of the solve are re-derived from a
+  // fresh opaque thread index after it (see line_kernel_tile_f32_fast), so no
+  // offset is held, and spilled, across the solve.
+  constexpr bool FRESH = (R >= 16);
 
   const NtPlanK& P = args.plan;
   int tid = threadIdx.x;                         // refreshed per tile (see line_kernel_tile)
@@ -341,8 +347,10 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
   const int64_t estride = args.estride;
   auto at = [&](int row, int el) -> int { return row * EP + el + (el / R) * CP; };
   auto base_of = [&](int64_t t) -> int64_t {
-    const int64_t o = t / args.tiles;
-    return o * args.ostride + (t - o * args.tiles) * TS;
+    int64_t tiles = args.tiles;
+    if constexpr (FRESH) asm volatile("" : "+s"(tiles));   // recomputed per tile, not held
+    const int64_t o = t / tiles;
+    return o * args.ostride + (t - o * tiles) * TS;
   };
 
   nt_dbl2 pf[PFV];
@@ -368,14 +376,16 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
 #pragma unroll 1
     for (int h = 0; h < LPW; ++h) {
       const int ql = ql0 + h * WAVES;            // tile-local line of this wavefront
+      const int qr = FRESH ? (tid >> 6) + h * WAVES : ql;
+      const int er = FRESH ? (tid & 63) * R : my_e0;
       double f[R];
 #pragma unroll
-      for (int k = 0; k < R; ++k) f[k] = lds[at(ql, my_e0 + k)];
+      for (int k = 0; k < R; ++k) f[k] = lds[at(qr, er + k)];
       double d[R];
       if (PROD == P_COMPACT) {
         double bnd[6];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) bnd[i] = lds[at(ql, (i < 3) ? i : N + 2 - i)];
+        for (int i = 0; i < 6; ++i) bnd[i] = lds[at(qr, (i < 3) ? i : N + 2 - i)];
         const double fl = shup<W>(f[R - 1], 1);
         const double fr = shdn<W>(f[0], 1);
         compact_rhs_chunk<R, true>(f, fl, fr, bnd, my_e0, N, args.s1, args.s2, d);
@@ -386,8 +396,14 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile_fast(const LineAr
       double x[R];
       solve_chunk<R, W, PREC, true>(d, x, P, lane, args.sweeps);
       wave_lds_sync();                           // the solution overwrites the line just read
+      int q = ql, e0 = my_e0;
+      if constexpr (FRESH) {
+        tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (NT - 1);
+        q = (tid >> 6) + h * WAVES;
+        e0 = (tid & 63) * R;
+      }
 #pragma unroll
-      for (int k = 0; k < R; ++k) lds[at(ql, my_e0 + k)] = x[k];
+      for (int k = 0; k < R; ++k) lds[at(q, e0 + k)] = x[k];
     }
   };
   auto store = [&](int64_t base) {

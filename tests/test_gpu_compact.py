@@ -154,6 +154,28 @@ def test_z_lines_rows_6mib_apart(dev, shape, precision):
     del fd, out
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("shape,axis", [
+    ((8, 1024, 64), 1),       # 1024-point y-lines in 32 whole 16-line tiles (fast path, R = 16)
+    ((40, 1024, 128), 1),     # 320 tiles: more tiles than workgroups, so the tile loop turns
+    ((1024, 4, 64), 0),       # z-lines with rows 2 KiB apart (fast path)
+])
+def test_fp64_1024_point_strided_fast_path(dev, shape, axis, precision):
+    """FP64-stored 1024-point strided lines on the whole-tile fast path at 16 rows
+    per lane, in FP64 and with FP32 + one refinement sweep, equal the oracle."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.workloads import compact_field
+    f = compact_field(shape, seed=17)
+    n = shape[axis]
+    h = 1.0 / (n - 1)
+    want = compact.derivative_banded(f, h, axis)
+    fd = torch.tensor(f, device=dev)
+    out = torch.full_like(fd, float("nan"))
+    core.compact_derivative(Plan(n, COMPACT_COEFFS), fd, out, h, axis, precision, 1)
+    tol = FP64_TOL if precision == "fp64" else REF_TOL
+    assert rel_err(out.cpu().numpy(), want) < tol
+
+
 def test_reference_call_sequence_unfused(dev):
     """run-compact.py:57-90 verbatim on the drop-in modules: permute -> compute_rhs ->
     NearToeplitzSolver.solve -> permute, for x, y and z."""
