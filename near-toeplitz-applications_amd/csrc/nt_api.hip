@@ -43,7 +43,9 @@ namespace {
 // n = 2^k and n = 3 * 2^(k-2) (the other two extents multiples of 64), with
 // the 512- and 1024-point strided slots taken from whole 512^3 / 1024^3 fields
 // and the 2048-point strided slot from the ADI y half step (whose lines do not
-// fit the whole-tile kernel).
+// fit the whole-tile kernel).  FP32 + refinement on 512-point strided lines takes
+// the one-line-per-wavefront whole-tile kernel (512^3 d/dy 461 -> 436 us, d/dz
+// 569 -> 531 us against two lines per wavefront).
 int g_tune[4][2][13][2] = {
     // contiguous lines: 0 wave-private transpose, 1 direct chunk loads, 2 workgroup tile
     {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 1}, {1, 1}},
@@ -51,7 +53,7 @@ int g_tune[4][2][13][2] = {
     // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
     // 2 persistent whole-tile kernel, 3 the same with two lines per wavefront (two workgroups per CU)
     {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {2, 1}},
-     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 3}, {3, 3}, {3, 3}, {2, 2}, {2, 1}, {1, 1}}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {1, 1}}},
     // ADI x half step (contiguous lines), variants as layout 0: at 2048 points the
     // wave-private transpose, which forms the stencil from coalesced reads of the
     // neighbouring lines (ADI 2048^2 step 58.7 -> 52.3 us; the compact derivative
