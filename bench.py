@@ -10,8 +10,9 @@ Metric: grid points per second and the fraction of HBM peak.
 
 Prints ONE JSON line.  Other configurations are timed after the headline and
 reported under "extras": d/dy and d/dz ("axes"), FP32 + refinement sweeps
-("refine"), ADI 2048^2 x 200 steps in a hipGraph ("adi"), a 1024^3 field
-stored in FP32 ("fp32_storage"), 64 mixed-size grids in one persistent launch
+along x and one sweep along y and z ("refine"), ADI 2048^2 x 200 steps in a
+hipGraph ("adi"), a 1024^3 field stored in FP32 ("fp32_storage"), 64
+mixed-size grids in one persistent launch
 ("batch"), and a Poisson request stream served with continuous batching
 ("serving").  --quick skips them; --case runs only the named ones.
 """
@@ -351,6 +352,18 @@ def main():
             extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
             extras[name]["max_rel_err_vs_fp64"] = err
             log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, err {err:.2e}")
+        # the strided axes with one sweep, against the FP64 kernel of the same axis
+        from ntamd import COMPACT_COEFFS, Plan, core
+        ref = torch.empty_like(out)
+        for name, axis in (("ddy_fp32_refine1", 1), ("ddz_fp32_refine1", 0)):
+            core.compact_derivative(Plan(N, COMPACT_COEFFS), f64, ref, 1.0 / (N - 1), axis, "fp64", 1)
+            t, _, o2 = compact_case(N, axis, "fp32_refine", 1, torch.float64, short, 3, field=f64)
+            err = ((o2 - ref).abs().max() / ref.abs().max()).item()
+            extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
+            extras[name]["max_rel_err_vs_fp64"] = err
+            log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, err {err:.2e}")
+            del o2
+        del ref
     del out, f64
     torch.cuda.empty_cache()
     if "adi" in cases:
