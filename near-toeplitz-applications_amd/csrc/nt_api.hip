@@ -44,8 +44,10 @@ namespace {
 // the 512- and 1024-point strided slots taken from whole 512^3 / 1024^3 fields
 // and the 2048-point strided slot from the ADI y half step (whose lines do not
 // fit the whole-tile kernel).  FP32 + refinement on 512-point strided lines takes
-// the one-line-per-wavefront whole-tile kernel (512^3 d/dy 461 -> 436 us, d/dz
-// 569 -> 531 us against two lines per wavefront).
+// the one-line-per-wavefront whole-tile kernel: against two lines per wavefront,
+// 512^3 d/dz 569 -> 531 and 585 -> 554 us in two processes, d/dy 461 -> 436 in
+// the first and 483 -> 538 us in the second (these shapes vary with where the
+// buffers land).
 int g_tune[4][2][13][2] = {
     // contiguous lines: 0 wave-private transpose, 1 direct chunk loads, 2 workgroup tile
     {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 1}, {1, 1}},
