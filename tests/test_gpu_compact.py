@@ -129,6 +129,46 @@ def test_fp32_storage_z_lines_far_rows(dev, shape, precision):
     del fd, out
 
 
+@pytest.fixture(scope="module")
+def field_1024_fp32(dev):
+    from ntamd.workloads import compact_field_device
+    f = compact_field_device(1024, seed=0, dtype=torch.float32, device=dev)
+    yield f
+    del f
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("precision,sweeps", [("fp64", 1), ("fp32_refine", 1), ("fp32_refine", 2)])
+@pytest.mark.parametrize("axis", [2, 1, 0])
+def test_1024cubed_fp32_storage_sampled_lines(dev, field_1024_fp32, axis, precision, sweeps):
+    """The full 1024^3 FP32-stored field (4 GiB, 1,048,576 lines per direction):
+    256 lines drawn at random along each axis equal the oracle's derivative of
+    the same FP32 values in FP64, and no point of the result is left unwritten."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    N = 1024
+    h = 1.0 / (N - 1)
+    fd = field_1024_fp32
+    out = torch.full(fd.shape, float("nan"), dtype=torch.float64, device=dev)
+    core.compact_derivative(Plan(N, COMPACT_COEFFS), fd, out, h, axis, precision, sweeps)
+    assert not torch.isnan(out).any()
+    rng = np.random.default_rng(40 + axis)
+    i = torch.tensor(rng.integers(0, N, 256), device=dev)
+    j = torch.tensor(rng.integers(0, N, 256), device=dev)
+    idx = [i, j]
+    idx.insert(axis, slice(None))
+    # the gathered lines are (256, N) for axes 2 and 1 (separated index arrays
+    # put their dimension first) and (N, 256) for axis 0: make them rows
+    src = fd[tuple(idx)].double().cpu().numpy()
+    got = out[tuple(idx)].cpu().numpy()
+    if src.shape == (N, 256):
+        src, got = src.T.copy(), got.T.copy()
+    assert src.shape == (256, N)
+    want = compact.derivative_banded(src[None], h, 2)[0]
+    tol = FP64_TOL if precision == "fp64" else REF_TOL
+    assert rel_err(got, want) < tol
+    del out
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
 @pytest.mark.parametrize("shape", [(1024, 1024, 768), (1024, 999, 801)])
 def test_z_lines_rows_6mib_apart(dev, shape, precision):

@@ -76,6 +76,42 @@ def test_every_strided_variant_fp32_storage(dev, precision, n, axis):
         _set(1, precision, n, keep)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("n", [64, 96, 99, 100, 128, 256, 512, 768, 1024, 2048])
+def test_every_contiguous_variant_fp32_storage(dev, precision, n):
+    """FP32-stored fields along contiguous x-lines, shape (3, 40, n): every
+    contiguous launch variant (wave-coalesced, direct-chunk and staged kernels,
+    with FP32 input rows; n = 99 leaves every row unaligned to 16 bytes) equals
+    the oracle's derivative of the same FP32 values in FP64.  This is the
+    reference's RHS + CR path (compact/rhs.py:21-36, neato.py:34-123) on FP32
+    storage."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.autotune import _set, current
+    from ntamd.workloads import compact_field
+    shape = (3, 40, n)
+    f32 = compact_field(shape, seed=n + 11).astype(np.float32)
+    h = 1.0 / (n - 1)
+    want = compact.derivative_banded(f32.astype(np.float64), h, 2)
+    fd = torch.tensor(f32, device=dev)
+    plan = Plan(n, COMPACT_COEFFS)
+    keep, nv = current(0, n, precision)
+    outs = []
+    try:
+        for v in range(nv):
+            _set(0, precision, n, v)
+            out = torch.full(shape, float("nan"), dtype=torch.float64, device=dev)
+            core.compact_derivative(plan, fd, out, h, 2, precision)
+            got = out.cpu().numpy()
+            err = np.abs(got - want).max() / np.abs(want).max()
+            assert err < TOL[precision], (v, err)
+            outs.append(got)
+    finally:
+        _set(0, precision, n, keep)
+    scale = np.abs(want).max()
+    for v in range(1, nv):
+        assert np.abs(outs[v] - outs[0]).max() / scale < TOL[precision], v
+
+
 @pytest.mark.parametrize("N", [34, 130, 256, 300, 512, 700, 1024, 2048])
 @pytest.mark.parametrize("layout", [2, 3])
 def test_every_variant_adi(dev, N, layout):
