@@ -1,23 +1,31 @@
 """Benchmark of the near-Toeplitz hot path on one MI355X.
 
 Headline workload: 4th-order compact d/dx of a 512^3 FP64 field
-f = sin(2 pi x) sin(2 pi y) sin(2 pi z) + N(0, 1e-3) noise (seed 0), i.e.
-262,144 near-Toeplitz systems of size 512, one fused launch per step (RHS
-stencil + cyclic reduction, each point read once and written once).
-Metric: grid points per second and the fraction of HBM peak.
+f = sin(2 pi x) sin(2 pi y) sin(2 pi z) + N(0, 1e-3) noise (numpy
+default_rng(0), generated on the host and copied to the device), i.e. 262,144
+near-Toeplitz systems of size 512, one fused launch per step (RHS stencil +
+cyclic reduction, each point read once and written once).  Metric: grid points
+per second and the fraction of HBM peak.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick] [--case NAME ...]
 
-Prints ONE JSON line.  Other configurations are timed after the headline and
-reported under "extras": d/dy and d/dz ("axes"), FP32 + refinement sweeps
-along x and one sweep along y and z ("refine"), ADI 2048^2 x 200 steps in a
-hipGraph ("adi"), a 1024^3 field stored in FP32 ("fp32_storage"), 64
-mixed-size grids in one persistent launch
-("batch"), and a Poisson request stream served with continuous batching
-("serving").  --quick skips them; --case runs only the named ones.
+Prints ONE compact JSON line on stdout (a few KB).  Every figure it carries
+is also in a longer record written to stderr.  Other configurations are timed
+after the headline and summarised under "extras": d/dy and d/dz ("axes"),
+FP32 + refinement sweeps ("refine"), ADI 2048^2 x 200 steps in a hipGraph and
+the partitioned ADI form ("adi"), a 1024^3 field stored in FP32
+("fp32_storage"), 64 mixed-size grids in one persistent launch ("batch"), and
+a Poisson request stream served with continuous batching ("serving").
+--quick skips them; --case runs only the named ones.
+
+Every result's error is measured against the CPU oracle (SciPy banded solve
+of the same matrix, oracle/) on lines copied from the same device field, with
+the sample size stated.  The CPU baseline times that oracle path on lines of
+the benchmarked field, on one core and on every host core.
 """
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -31,11 +39,198 @@ for _p in (ROOT, PKG):
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "compact_ddx_512_fp64_traffic.json")
+METRIC = "Grid pts/s & % HBM peak, 512^3 compact d/dx; p50/p99 served vs plain FP64"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
+
+def sig(x, digits=4):
+    """Round floats to `digits` significant digits, recursively (compact JSON)."""
+    if isinstance(x, float):
+        if x == 0.0 or not math.isfinite(x):
+            return x
+        return float(f"{x:.{digits}g}")
+    if isinstance(x, dict):
+        return {k: sig(v, digits) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [sig(v, digits) for v in x]
+    return x
+
+
+def summary(values):
+    """{max, p50, count} of a list of per-grid errors."""
+    if not values:
+        return None
+    return {"max": max(values), "p50": statistics.median(values), "count": len(values)}
+
+
+# ---------------------------------------------------------------------------
+# oracle checks (the CPU oracle is the checker here, never the thing timed on GPU)
+
+def gather_lines(t, axis, idx_a, idx_b):
+    """Lines along `axis` of a 3-D torch tensor at the index pairs (idx_a, idx_b)
+    of the other two axes, in order, as a host float64 array (lines, n)."""
+    import torch
+    a = torch.as_tensor(idx_a, device=t.device)
+    b = torch.as_tensor(idx_b, device=t.device)
+    if axis == 2:
+        g = t[a, b, :]
+    elif axis == 1:
+        g = t[a, :, b]          # separated index arrays: their dimension comes first
+    else:
+        g = t[:, a, b].t()
+    return g.double().cpu().numpy()
+
+
+def oracle_err(f, out, axis, h, nlines=256, seed=0):
+    """Max relative error of `out` against the oracle's derivative of `f` on
+    `nlines` lines drawn at random (numpy default_rng(seed)) along `axis`."""
+    import numpy as np
+    from oracle.compact import COMPACT_COEFFS, compact_rhs
+    from oracle.cr import banded_solve
+    others = [d for d in range(3) if d != axis]
+    rng = np.random.default_rng(seed)
+    ia = rng.integers(0, f.shape[others[0]], nlines)
+    ib = rng.integers(0, f.shape[others[1]], nlines)
+    src = gather_lines(f, axis, ia, ib)
+    got = gather_lines(out, axis, ia, ib)
+    want = banded_solve(COMPACT_COEFFS, compact_rhs(src, h))
+    return float(np.abs(got - want).max() / np.abs(want).max())
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: NumPy RHS + one scipy.linalg.solve_banded call (LAPACK gtsv),
+# the reference's CPU path (parallelcompact/misc.py:5-19), on lines of the
+# benchmarked field.
+
+def _pool_init():
+    from threadpoolctl import threadpool_limits
+    threadpool_limits(1)
+    globals()["_POOL_LIMIT"] = True
+
+
+def _pool_task(args):
+    """Worker: the compact derivative of lines lo..hi of a shared (lines, n)
+    array into the shared output; returns the seconds it took."""
+    import numpy as np
+    from multiprocessing import shared_memory
+    from oracle.compact import COMPACT_COEFFS, compact_rhs
+    from oracle.cr import banded_solve
+    name_in, name_out, shape, lo, hi, h = args
+    si = shared_memory.SharedMemory(name=name_in)
+    so = shared_memory.SharedMemory(name=name_out)
+    try:
+        src = np.ndarray(shape, dtype=np.float64, buffer=si.buf)
+        dst = np.ndarray(shape, dtype=np.float64, buffer=so.buf)
+        t0 = time.perf_counter()
+        dst[lo:hi] = banded_solve(COMPACT_COEFFS, compact_rhs(src[lo:hi], h))
+        return time.perf_counter() - t0
+    finally:
+        del src, dst
+        si.close()
+        so.close()
+
+
+def host_lines(f, axis, nlines, seed):
+    """`nlines` lines along `axis` of the host field f, as a (nlines, n) array."""
+    import numpy as np
+    others = [d for d in range(3) if d != axis]
+    rng = np.random.default_rng(seed)
+    ia = rng.integers(0, f.shape[others[0]], nlines)
+    ib = rng.integers(0, f.shape[others[1]], nlines)
+    if axis == 2:
+        return np.ascontiguousarray(f[ia, ib, :])
+    if axis == 1:
+        return np.ascontiguousarray(f[ia, :, ib])
+    return np.ascontiguousarray(f[:, ia, ib].T)
+
+
+def cpu_baseline(f, h, sample_lines=65536, reps=3, pool_workers=None):
+    """The oracle path timed on the host: per direction, `sample_lines` lines of
+    the benchmarked field f on one core (BLAS limited to one thread), and d/dx
+    of every x-line of f split over a process pool of every available host core
+    (at most 16).  Runs before the GPU is touched: the pool is started as
+    child processes of a process that has not initialised HIP."""
+    import numpy as np
+    from threadpoolctl import threadpool_info, threadpool_limits
+    from oracle.compact import COMPACT_COEFFS, compact_rhs
+    from oracle.cr import banded_solve
+    n = f.shape[2]
+    single = {}
+    with threadpool_limits(1):
+        blas_threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+        for axis, name in ((2, "ddx"), (1, "ddy"), (0, "ddz")):
+            lines = host_lines(f, axis, sample_lines, seed=70 + axis)
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                banded_solve(COMPACT_COEFFS, compact_rhs(lines, h))
+                times.append(time.perf_counter() - t0)
+            med = statistics.median(times)
+            single[name] = sample_lines * n / med
+            log(f"[bench] CPU 1 core {name}: {single[name]:.3e} pts/s "
+                f"({sample_lines} lines, median of {reps})")
+    out = {"value": single["ddx"], "unit": "grid pts/s", "cores": 1, "kind": "port",
+           "sample": f"{sample_lines} lines per direction copied from the benchmarked {n}^3 field "
+                     f"(seeded draw), NumPy RHS + one scipy.linalg.solve_banded call (LAPACK gtsv), "
+                     f"BLAS limited to 1 thread, median of {reps}",
+           "blas_threads": blas_threads, "host_cores_available": len(os.sched_getaffinity(0)),
+           "single_core_pts_per_s": single, "all_cores": None}
+    workers = pool_workers or min(16, len(os.sched_getaffinity(0)))
+    if workers > 1:
+        try:
+            out["all_cores"] = cpu_pool(f, h, workers, reps)
+        except Exception as e:          # the all-core leg is optional
+            log(f"[bench] CPU pool leg skipped: {e!r}")
+    if out["all_cores"] is not None:
+        ac = out["all_cores"]
+        out.update(value=ac["pts_per_s"], cores=workers,
+                   sample=f"d/dx of all {f.shape[0] * f.shape[1]} x-lines of the benchmarked {n}^3 field over "
+                          f"{workers} processes x 1 thread (NumPy RHS + solve_banded, LAPACK gtsv), median "
+                          f"of {reps}; single_core: {sample_lines} lines per direction, 1 thread")
+    return out
+
+
+def cpu_pool(f, h, workers, reps):
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from multiprocessing import shared_memory
+    import numpy as np
+    nl = f.shape[0] * f.shape[1]
+    n = f.shape[2]
+    shape = (nl, n)
+    si = shared_memory.SharedMemory(create=True, size=nl * n * 8)
+    so = shared_memory.SharedMemory(create=True, size=nl * n * 8)
+    try:
+        np.ndarray(shape, dtype=np.float64, buffer=si.buf)[:] = f.reshape(shape)
+        bounds = np.linspace(0, nl, workers + 1).astype(int)
+        tasks = [(si.name, so.name, shape, int(bounds[i]), int(bounds[i + 1]), h) for i in range(workers)]
+        with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn"),
+                                 initializer=_pool_init) as ex:
+            # one small task per worker first: interpreter start-up and imports are not timed
+            warm = [(si.name, so.name, shape, i * 4, i * 4 + 4, h) for i in range(workers)]
+            list(ex.map(_pool_task, warm))
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                list(ex.map(_pool_task, tasks, chunksize=1))
+                times.append(time.perf_counter() - t0)
+        med = statistics.median(times)
+        res = {"pts_per_s": nl * n / med, "seconds": med}
+        log(f"[bench] CPU {workers} cores d/dx: {res['pts_per_s']:.3e} pts/s (all {nl} x-lines, "
+            f"median of {reps})")
+        return res
+    finally:
+        si.close()
+        si.unlink()
+        so.close()
+        so.unlink()
+
+
+# ---------------------------------------------------------------------------
+# GPU cases
 
 def time_launches(fn, steps, warmup):
     """Warm up, then time `steps` launches: per-launch HIP events on the current
@@ -58,63 +253,36 @@ def time_launches(fn, steps, warmup):
             "min_s": min(per), "max_s": max(per)}
 
 
-def compact_case(N, axis, precision, sweeps, in_dtype, steps, warmup, field=None):
+def compact_case(f, axis, precision, sweeps, steps, warmup, out=None):
     import torch
     from ntamd import COMPACT_COEFFS, Plan, core
-    from ntamd.workloads import compact_field_device
-    f = field if field is not None else compact_field_device(N, seed=0, dtype=in_dtype)
-    out = torch.empty(f.shape, dtype=torch.float64, device=f.device)
-    plan = Plan(N, COMPACT_COEFFS)
-    h = 1.0 / (N - 1)
+    n = f.shape[axis]
+    if out is None:
+        out = torch.empty(f.shape, dtype=torch.float64, device=f.device)
+    plan = Plan(n, COMPACT_COEFFS)
+    h = 1.0 / (n - 1)
     t = time_launches(lambda: core.compact_derivative(plan, f, out, h, axis, precision, sweeps),
                       steps, warmup)
-    pts = N ** 3
+    pts = f.numel()
     nbytes = pts * (f.element_size() + 8)
     t["pts_per_s"] = pts / t["mean_s"]
     t["alg_bytes"] = nbytes
     t["gbps"] = nbytes / t["mean_s"] / 1e9
     t["hbm_frac"] = nbytes / t["mean_s"] / HBM_PEAK
-    return t, f, out
-
-
-def cpu_baseline(N, sample_lines, reps=3):
-    """SciPy banded solve (LAPACK gtsv) + NumPy RHS on a sample of the 512^3
-    d/dx workload's lines: the CPU path of the same matrices."""
-    import numpy as np
-    from oracle.compact import COMPACT_COEFFS, compact_rhs
-    from oracle.cr import banded_solve
-    rng = np.random.default_rng(0)
-    x = np.linspace(0.0, 1.0, N)
-    ys = rng.random(sample_lines)
-    zs = rng.random(sample_lines)
-    f = (np.sin(2 * np.pi * zs)[:, None] * np.sin(2 * np.pi * ys)[:, None]
-         * np.sin(2 * np.pi * x)[None, :]) + rng.normal(0.0, 1e-3, (sample_lines, N))
-    h = 1.0 / (N - 1)
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        banded_solve(COMPACT_COEFFS, compact_rhs(f, h))
-        times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    try:
-        from threadpoolctl import threadpool_info
-        blas_threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:
-        blas_threads = None
-    return {"value": sample_lines * N / med, "unit": "grid pts/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{sample_lines} x-lines of the 512^3 d/dx shape, y and z drawn at random "
-                      f"({sample_lines * N} pts), NumPy RHS + one scipy.linalg.solve_banded "
-                      f"call (LAPACK gtsv, single-threaded), median of {reps}",
-            "seconds": med, "host_cores_available": len(os.sched_getaffinity(0)),
-            "blas_threads": blas_threads}
+    return t, out
 
 
 def adi_case(N, r, nsteps, reps):
+    """ADI 2048^2, 200 Peaceman-Rachford steps in one hipGraph (monolithic
+    solves), and the one-GPU partitioned form: 16 subdomains of 128 per line
+    plus the reduced 32-unknown system for both half steps."""
+    import numpy as np
     import torch
     from adi.solver import ADISolver
     from ntamd.workloads import gaussian_ic
-    u = torch.tensor(gaussian_ic(N, seed=0), device="cuda")
+    from oracle import adi as oadi
+    u0 = gaussian_ic(N, seed=0)
+    u = torch.tensor(u0, device="cuda")
     s = ADISolver(N, r)
     s.capture(u, nsteps)
     s.replay()
@@ -130,17 +298,51 @@ def adi_case(N, r, nsteps, reps):
         ts.append(a.elapsed_time(b) * 1e-3)
     t = statistics.median(ts)
     half_bytes = 16 * N * N
-    # the one-GPU partitioned form: 16 subdomains of 128 per line plus the reduced
-    # 32-unknown system, one launch per x-implicit half step
+    # one step from the initial condition against the oracle's PR step
+    u1 = torch.tensor(u0, device="cuda")
+    s1 = ADISolver(N, r)
+    s1.step(u1, 1)
+    want = oadi.pr_run(u0, r, 1)
+    err1 = float(np.abs(u1.cpu().numpy() - want).max() / np.abs(want).max())
+    res = {"N": N, "r": r, "steps": nsteps, "ms_per_step": 1e3 * t / nsteps,
+           "effective_gbps": 2 * nsteps * half_bytes / t / 1e9, "max_rel_err_vs_oracle_1step": err1,
+           "note": "32 MiB field stays in the 256 MB Infinity Cache: effective, not HBM, GB/s"}
     from ntamd import adi_coeffs
     from ntamd.partition import PartitionPlan
     pp = PartitionPlan(N, 16, adi_coeffs(r))
     w = torch.empty_like(u)
     tp = time_launches(lambda: pp.adi_half_x(u, w, r), 20, 3)
-    return {"N": N, "r": r, "steps": nsteps, "s_per_run": t, "ms_per_step": 1e3 * t / nsteps,
-            "effective_gbps": 2 * nsteps * half_bytes / t / 1e9,
-            "partitioned_x_half_step_16x128_us": tp["mean_s"] * 1e6,
-            "note": "32 MiB field stays in the 256 MB Infinity Cache: effective, not HBM, GB/s"}
+    res["partitioned_x_half_step_16x128_us"] = tp["mean_s"] * 1e6
+    if hasattr(pp, "capture_steps"):
+        res.update(partitioned_adi_case(pp, u0, r, nsteps, reps))
+    return res
+
+
+def partitioned_adi_case(pp, u0, r, nsteps, reps):
+    """Both half steps partitioned (16 x 128 + reduced 32 per line), all
+    `nsteps` steps in one hipGraph; error against the oracle after one step."""
+    import numpy as np
+    import torch
+    from oracle import adi as oadi
+    u = torch.tensor(u0, device="cuda")
+    g = pp.capture_steps(u, nsteps)
+    g.replay()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b) * 1e-3)
+    t = statistics.median(ts)
+    u1 = torch.tensor(u0, device="cuda")
+    pp.adi_steps(u1, 1)
+    want = oadi.pr_run(u0, r, 1)
+    err = float(np.abs(u1.cpu().numpy() - want).max() / np.abs(want).max())
+    return {"partitioned_ms_per_step": 1e3 * t / nsteps, "partitioned_max_rel_err_vs_oracle_1step": err}
 
 
 def measured_traffic():
@@ -172,51 +374,47 @@ def pmc_probe():
 
 def config4_case(N=1024, reps=5):
     """1024^3 field stored in FP32 (4 GiB): d/dx, d/dy, d/dz with the FP32 solve +
-    one FP64-residual sweep, against plain FP64 (FP64-stored field, FP64 solve).
-    Errors are relative to the FP64 solve of the same (FP32-stored) values, which
-    is timed too: FP32 storage with FP64 arithmetic."""
+    one FP64-residual sweep, against plain FP64 (the FP64-stored field, FP64
+    solve), and FP32 storage with the FP64 solve.  Errors of every form are
+    against the oracle on 256 lines per axis copied from the device field."""
     import torch
-    from ntamd import COMPACT_COEFFS, Plan, core
     from ntamd.workloads import compact_field_device
     # Blocks cached from the earlier cases go back to the driver first, so the
     # 1024^3 fields are fresh allocations.
     torch.cuda.empty_cache()
     f32 = compact_field_device(N, seed=0, dtype=torch.float32)
     f64 = f32.double()
-    out_r = torch.empty((N, N, N), dtype=torch.float64, device=f32.device)
-    out_64 = torch.empty_like(out_r)
-    plan = Plan(N, COMPACT_COEFFS)
+    out = torch.empty((N, N, N), dtype=torch.float64, device=f32.device)
     h = 1.0 / (N - 1)
     res = {}
     for axis, name in ((2, "ddx"), (1, "ddy"), (0, "ddz")):
-        tr = time_launches(lambda: core.compact_derivative(plan, f32, out_r, h, axis, "fp32_refine", 1),
-                           reps, 2)
-        t64 = time_launches(lambda: core.compact_derivative(plan, f64, out_64, h, axis, "fp64", 1), reps, 2)
-        ref64 = torch.empty_like(out_r)
-        # FP64 solve of the same FP32-stored values: the error reference, and
-        # timed as the third form (FP32 storage, FP64 arithmetic)
-        ts = time_launches(lambda: core.compact_derivative(plan, f32, ref64, h, axis, "fp64", 1), reps, 2)
-        err = ((out_r - ref64).abs().max() / ref64.abs().max()).item()
-        res[name] = {"fp32_refine_s": tr["mean_s"], "fp64_s": t64["mean_s"],
-                     "speedup_vs_fp64": t64["mean_s"] / tr["mean_s"], "max_rel_err_vs_fp64": err,
-                     "fp32_refine_gbps": N ** 3 * 12 / tr["mean_s"] / 1e9,
-                     "fp64_gbps": N ** 3 * 16 / t64["mean_s"] / 1e9,
-                     "fp32_storage_fp64_solve_s": ts["mean_s"],
-                     "fp32_storage_fp64_solve_speedup_vs_fp64": t64["mean_s"] / ts["mean_s"]}
-        log(f"[bench] 1024^3 {name}: fp32+ref {tr['mean_s'] * 1e3:.3f} ms, fp64 {t64['mean_s'] * 1e3:.3f} ms, "
-            f"x{res[name]['speedup_vs_fp64']:.2f}, err {err:.2e}; fp32 storage + fp64 solve "
-            f"{ts['mean_s'] * 1e3:.3f} ms, x{res[name]['fp32_storage_fp64_solve_speedup_vs_fp64']:.2f}")
-        del ref64
-    del f32, f64, out_r, out_64
+        tr, _ = compact_case(f32, axis, "fp32_refine", 1, reps, 2, out=out)
+        er = oracle_err(f32, out, axis, h, 256, seed=axis)
+        t64, _ = compact_case(f64, axis, "fp64", 1, reps, 2, out=out)
+        e64 = oracle_err(f64, out, axis, h, 256, seed=axis)
+        ts, _ = compact_case(f32, axis, "fp64", 1, reps, 2, out=out)
+        es = oracle_err(f32, out, axis, h, 256, seed=axis)
+        # ms and error vs the oracle of: FP32 storage + FP32 solve + 1 sweep ("fp32_refine"),
+        # FP64 storage + FP64 solve ("fp64"), FP32 storage + FP64 solve ("f32_store_fp64")
+        res[name] = {"ms": {"fp32_refine": tr["mean_s"] * 1e3, "fp64": t64["mean_s"] * 1e3,
+                            "f32_store_fp64": ts["mean_s"] * 1e3},
+                     "err": {"fp32_refine": er, "fp64": e64, "f32_store_fp64": es},
+                     "speedup_vs_fp64": t64["mean_s"] / tr["mean_s"]}
+        log(f"[bench] 1024^3 {name}: fp32+ref {tr['mean_s'] * 1e3:.3f} ms, fp64 {t64['mean_s'] * 1e3:.3f} ms "
+            f"(x{res[name]['speedup_vs_fp64']:.2f}); fp32 storage + fp64 solve {ts['mean_s'] * 1e3:.3f} ms; "
+            f"err vs oracle {er:.2e} / {e64:.2e} / {es:.2e}")
+    del f32, f64, out
     torch.cuda.empty_cache()
+    res["err_sample"] = "256 lines per axis and form"
     return res
 
 
-def mixed_batch_case(count=64, seed=1, reps=5, sizes=None, precision="fp32_refine"):
-    """64 grids, per-axis sizes uniform in {64, 96, ..., 512} (seeded), random
-    derivative direction each, FP32 solve + one sweep, all in one persistent launch,
-    against the same jobs as separate fused launches.  Per-grid max relative
-    error against the plain FP64 kernel."""
+def mixed_batch_case(count=64, seed=1, reps=5, sizes=None, precision="fp32_refine", lines_per_grid=32):
+    """`count` grids, per-axis sizes uniform in `sizes` (default {64, 96, ...,
+    512}, numpy default_rng(seed)), a random derivative direction each, FP32
+    solve + one sweep, all in one persistent launch, against the same jobs as
+    separate fused launches.  Per-grid max relative error against the oracle
+    on `lines_per_grid` lines of each grid."""
     import numpy as np
     import torch
     from ntamd import COMPACT_COEFFS, Plan, core
@@ -234,6 +432,8 @@ def mixed_batch_case(count=64, seed=1, reps=5, sizes=None, precision="fp32_refin
         pts += int(np.prod(shape))
     b = MixedBatch(jobs)
     t = time_launches(b.run, reps, 2)
+    errs = [oracle_err(j["f"], j["out"], j["axis"], j["dx"], lines_per_grid, seed=k)
+            for k, j in enumerate(jobs)]
     plans = {n: Plan(n, COMPACT_COEFFS) for n in {j["f"].shape[j["axis"]] for j in jobs}}
 
     def separate():
@@ -241,29 +441,27 @@ def mixed_batch_case(count=64, seed=1, reps=5, sizes=None, precision="fp32_refin
             core.compact_derivative(plans[j["f"].shape[j["axis"]]], j["f"], j["out"], j["dx"], j["axis"],
                                     j["precision"], j["sweeps"])
     ts = time_launches(separate, reps, 2)
-    errs = []
-    for j in jobs:
-        ref = torch.empty_like(j["out"])
-        n = j["f"].shape[j["axis"]]
-        core.compact_derivative(Plan(n, COMPACT_COEFFS), j["f"], ref, j["dx"], j["axis"], "fp64", 1)
-        errs.append(((j["out"] - ref).abs().max() / ref.abs().max()).item())
     # medians: a launch sequence that pays a one-off cost (a code object loaded,
     # an occupancy query answered the first time) does not set the comparison
-    res = {"grids": count, "points": pts, "launch_s": t["median_s"], "pts_per_s": pts / t["median_s"],
-           "gbps": pts * 16 / t["median_s"] / 1e9, "separate_launches_s": ts["median_s"],
-           "launch_s_min_max": [t["min_s"], t["max_s"]], "separate_s_min_max": [ts["min_s"], ts["max_s"]],
-           "max_rel_err_vs_fp64_per_grid_max": max(errs),
-           "max_rel_err_vs_fp64_per_grid": errs}
+    res = {"grids": count, "points": pts, "launch_ms": t["median_s"] * 1e3,
+           "gbps": pts * 16 / t["median_s"] / 1e9, "separate_launches_ms": ts["median_s"] * 1e3,
+           "speedup_vs_separate": ts["median_s"] / t["median_s"],
+           "err_vs_oracle_per_grid": summary(errs), "err_lines_per_grid": lines_per_grid}
     log(f"[bench] mixed batch: {count} grids, {pts / 1e6:.0f} M pts, {t['median_s'] * 1e3:.3f} ms "
         f"(min {t['min_s'] * 1e3:.3f}, max {t['max_s'] * 1e3:.3f}; {res['gbps']:.0f} GB/s); {count} separate "
         f"launches {ts['median_s'] * 1e3:.3f} ms (min {ts['min_s'] * 1e3:.3f}, max {ts['max_s'] * 1e3:.3f}); "
-        f"worst grid err {max(errs):.2e}")
+        f"worst grid err vs oracle {max(errs):.2e}")
+    if count <= 64:
+        log(f"[bench] mixed batch per-grid errors vs oracle: {[float(f'{e:.3g}') for e in errs]}")
     del jobs, b
     torch.cuda.empty_cache()
     return res
 
 
-def serving_case(count=1000, rates=(500.0, 2000.0), seed=2):
+SERVE_RATES = (500.0, 2000.0, 3000.0)
+
+
+def serving_case(count=1000, rates=SERVE_RATES, seed=2):
     """Poisson arrivals (rate req/s, numpy default_rng(2)): compact derivatives
     of random-phase trig fields (per-axis sizes U[32,512], random axis) and ADI
     runs (side U[256,2048], 1-20 steps, r U[0.1,2]), FP64 or FP32+refinement
@@ -319,7 +517,25 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.pmc_probe:
+        import torch
+        from ntamd import _lib
+        _lib.load()
+        pmc_probe()
+        return
+
     import numpy as np
+    from ntamd.workloads import compact_field
+    N = 512
+    h = 1.0 / (N - 1)
+    t0 = time.perf_counter()
+    f_host = compact_field(N, seed=0)
+    log(f"[bench] 512^3 field generated on the host in {time.perf_counter() - t0:.1f} s")
+    cpu = None
+    if not (args.no_cpu or args.quick):
+        # before the GPU is initialised: the all-core leg starts child processes
+        cpu = cpu_baseline(f_host, h)
+
     import torch
     from ntamd import _lib, autotune
     _lib.load()
@@ -327,67 +543,55 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
     tuned = autotune.apply()          # launch shapes chosen by the autotuner, if cached
+    f64 = torch.from_numpy(f_host).to("cuda")
+    del f_host
 
-    if args.pmc_probe:
-        pmc_probe()
-        return
-
-    N = 512
     log(f"[bench] headline: compact d/dx {N}^3 FP64, steps={args.steps} warmup={args.warmup}")
-    head, f64, out = compact_case(N, 2, "fp64", 1, torch.float64, args.steps, args.warmup)
-    log(f"[bench] d/dx: {head['mean_s'] * 1e6:.1f} us/launch, {head['gbps']:.0f} GB/s")
+    head, out = compact_case(f64, 2, "fp64", 1, args.steps, args.warmup)
+    head_err = oracle_err(f64, out, 2, h, 1024, seed=0)
+    log(f"[bench] d/dx: {head['mean_s'] * 1e6:.1f} us/launch, {head['gbps']:.0f} GB/s, "
+        f"err vs oracle {head_err:.2e} (1024 lines)")
 
     extras = {}
     cases = set(args.case) if args.case else (set() if args.quick else set(EXTRA_CASES))
     short = max(10, args.steps // 2)
+    o2 = torch.empty_like(out)
+
+    def line(t, err):
+        return {"us": t["mean_s"] * 1e6, "gbps": t["gbps"], "hbm_frac": t["hbm_frac"], "err_vs_oracle": err}
     if "axes" in cases:
         for name, axis in (("ddy_fp64", 1), ("ddz_fp64", 0)):
-            t, _, _ = compact_case(N, axis, "fp64", 1, torch.float64, short, 3, field=f64)
-            extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
-            log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, {t['gbps']:.0f} GB/s")
+            t, _ = compact_case(f64, axis, "fp64", 1, short, 3, out=o2)
+            extras[name] = line(t, oracle_err(f64, o2, axis, h, 256, seed=axis))
+            log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, {t['gbps']:.0f} GB/s, "
+                f"err {extras[name]['err_vs_oracle']:.2e}")
     if "refine" in cases:
-        for name, sweeps in (("ddx_fp32_refine1", 1), ("ddx_fp32_refine2", 2)):
-            t, _, o2 = compact_case(N, 2, "fp32_refine", sweeps, torch.float64, short, 3, field=f64)
-            err = ((o2 - out).abs().max() / out.abs().max()).item()
-            extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
-            extras[name]["max_rel_err_vs_fp64"] = err
-            log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, err {err:.2e}")
-        # the strided axes with one sweep, against the FP64 kernel of the same axis
-        from ntamd import COMPACT_COEFFS, Plan, core
-        ref = torch.empty_like(out)
-        for name, axis in (("ddy_fp32_refine1", 1), ("ddz_fp32_refine1", 0)):
-            core.compact_derivative(Plan(N, COMPACT_COEFFS), f64, ref, 1.0 / (N - 1), axis, "fp64", 1)
-            t, _, o2 = compact_case(N, axis, "fp32_refine", 1, torch.float64, short, 3, field=f64)
-            err = ((o2 - ref).abs().max() / ref.abs().max()).item()
-            extras[name] = {k: t[k] for k in ("mean_s", "pts_per_s", "gbps", "hbm_frac")}
-            extras[name]["max_rel_err_vs_fp64"] = err
-            log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, err {err:.2e}")
-            del o2
-        del ref
-    del out, f64
+        for name, axis, sweeps in (("ddx_fp32_refine1", 2, 1), ("ddx_fp32_refine2", 2, 2),
+                                   ("ddy_fp32_refine1", 1, 1), ("ddz_fp32_refine1", 0, 1)):
+            t, _ = compact_case(f64, axis, "fp32_refine", sweeps, short, 3, out=o2)
+            extras[name] = line(t, oracle_err(f64, o2, axis, h, 256, seed=10 + axis))
+            log(f"[bench] {name}: {t['mean_s'] * 1e6:.1f} us, err {extras[name]['err_vs_oracle']:.2e}")
+    del out, o2, f64
     torch.cuda.empty_cache()
     if "adi" in cases:
         a = adi_case(2048, 0.5, 200, 3)
         extras["adi_2048_200steps_graph"] = a
-        log(f"[bench] ADI 2048^2: {a['ms_per_step']:.3f} ms/step; partitioned x half step "
-            f"(16 x 128 + reduced 32): {a['partitioned_x_half_step_16x128_us']:.1f} us")
+        log(f"[bench] ADI 2048^2: {a['ms_per_step']:.3f} ms/step (err 1 step {a['max_rel_err_vs_oracle_1step']:.1e}); "
+            f"partitioned x half step (16 x 128 + reduced 32): {a['partitioned_x_half_step_16x128_us']:.1f} us"
+            + (f"; partitioned steps {a['partitioned_ms_per_step']:.3f} ms/step"
+               if "partitioned_ms_per_step" in a else ""))
     if "fp32_storage" in cases:
         extras["fp32_storage_1024"] = config4_case()
     if "batch" in cases:
         extras["mixed_batch_64"] = mixed_batch_case()
-        extras["mixed_batch_small_1024"] = mixed_batch_case(count=1024, sizes=np.arange(16, 65, 8), reps=5)
+        extras["mixed_batch_small_1024"] = mixed_batch_case(count=1024, sizes=np.arange(16, 65, 8), reps=5,
+                                                            lines_per_grid=8)
     if "serving" in cases:
         extras["serving"] = serving_case()
 
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] CPU baseline (SciPy banded) ...")
-        cpu = cpu_baseline(N, 262144)     # every x-line of the 512^3 workload, ~3 s per repetition
-        log(f"[bench] CPU: {cpu['value']:.3e} pts/s")
-
     pts = N ** 3
     rec = {
-        "metric": "Grid pts/s & % HBM peak, 512^3 compact d/dx; p50/p99 served vs plain FP64",
+        "metric": METRIC,
         "value": pts / head["mean_s"],
         "unit": "grid pts/s",
         "n_gpus": 1,
@@ -398,23 +602,25 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (sin(2pi x)sin(2pi y)sin(2pi z) + N(0,1e-3), torch seed 0, on device)",
+        "data": "synthetic (sin(2pi x)sin(2pi y)sin(2pi z) + N(0,1e-3), numpy default_rng(0), host-made)",
         "config": {"workload": "compact 4th-order d/dx, 512^3 FP64, 262144 systems x 512",
-                   "n": N, "systems": N * N, "coeffs": [1, 2, 0.25, 1, 0.25, 2, 1],
-                   "precision": "fp64"},
+                   "n": N, "systems": N * N, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": head["gbps"], "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": head["hbm_frac"], "traffic": measured_traffic(),
-                     "traffic_source": "profiles/compact_ddx_512_fp64.md (rocprofv3 --pmc)",
-                     "alg_bytes_per_launch": head["alg_bytes"],
-                     "kernel_mean_s": head["mean_s"], "kernel_median_s": head["median_s"]},
-        "hbm_peak_frac": head["hbm_frac"],
+                     "alg_bytes_per_launch": head["alg_bytes"], "kernel_mean_us": head["mean_s"] * 1e6},
+        "max_rel_err_vs_oracle": head_err,
+        "err_sample": "1024 x-lines of the same device field, oracle = SciPy banded (LAPACK gtsv)",
         "served": served_summary(extras.get("serving")),
         "cpu_baseline": cpu,
-        "wall_s_timed_region": head["wall_s"],
-        "extras": extras,
-        "tuning": tuned,
+        "extras": {k: v for k, v in extras.items() if k != "serving"},
     }
-    print(json.dumps(rec), flush=True)
+    if tuned:
+        rec["tuning"] = tuned
+    full = json.dumps(rec)
+    log(f"[bench] detail {full}")
+    small = json.dumps(sig(rec), separators=(",", ":"))
+    log(f"[bench] stdout line: {len(small)} bytes")
+    print(small, flush=True)
 
 
 if __name__ == "__main__":
