@@ -15,6 +15,7 @@
 #ifndef NTOEPLITZ_H
 #define NTOEPLITZ_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -197,9 +198,28 @@ typedef struct nt_batch_job {
 size_t nt_batch_image_bytes(int njobs, int nplans);
 int nt_batch_pack(const nt_batch_job* jobs, int njobs, const nt_plan* const* plans, int nplans,
                   void* host_image, size_t bytes, int64_t* total_tiles);
-/* grid <= 0: two workgroups per compute unit */
+/* grid <= 0: four workgroups per compute unit */
 int nt_batch_launch(const void* dev_image, int njobs, int nplans, int64_t total_tiles,
                     unsigned int* dev_counter, int grid, void* stream);
+
+/* ---- allocation-free batch ----------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device or
+ * pinned host memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same jobs and kernel as
+ * nt_batch_pack + nt_batch_launch, with the job image and work counter kept in
+ * a fixed 1 MiB device array of the library (one per GPU, held until the process
+ * ends).  The image is packed on the host into a static buffer and carried to
+ * the device in the arguments of small fill kernels on `stream`, so nothing is
+ * allocated and no host memory needs pinning; the call returns once the work is
+ * enqueued.  Results are identical to nt_batch_launch's.
+ * Limits: nt_batch_image_bytes(njobs, nplans) <= NT_BATCH_NOALLOC_IMAGE_MAX
+ * (NT_EINVAL beyond it).  `stream` must belong to the current device.  Every
+ * call on one GPU shares that GPU's array: calls on the same stream are
+ * ordered, but calls that overlap on one GPU (other streams, other host
+ * threads enqueueing while earlier work still runs) race on it. */
+#define NT_BATCH_NOALLOC_IMAGE_MAX ((size_t)1048320)   /* 1 MiB minus the 256-byte counter block */
+int nt_batch_run_noalloc(const nt_batch_job* jobs, int njobs, const nt_plan* const* plans, int nplans,
+                         int grid, void* stream);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/ntoeplitz.h"
 #include "nt_error.h"
@@ -253,12 +254,93 @@ __global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
 
 }  // namespace nt
 
+// ---- the allocation-free batch (nt_batch_run_noalloc) ---------------------------
+// Its job image and work counter live in `scratch`, a file-scope device array
+// of fixed size: each GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` names a host-side stand-in, never the
+// device memory: kernels get the per-device address hipGetSymbolAddress gives.
+// Layout: bytes 0..255 the work counter (and padding), then the image.
+constexpr size_t NOALLOC_HEAD = 256;
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_HEAD + NT_BATCH_NOALLOC_IMAGE_MAX;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace nt {
+
+// One chunk of the host-packed image, carried to the device by value in the
+// kernel arguments of a fill launch.
+constexpr size_t FILL_BYTES = 2048;
+struct FillChunk {
+  unsigned int w[FILL_BYTES / 4];
+};
+
+__global__ void __launch_bounds__(256) scratch_fill_kernel(unsigned int* dst, const FillChunk c, int nwords,
+                                                           unsigned int* counter) {
+  for (int i = threadIdx.x; i < nwords; i += 256) dst[i] = c.w[i];
+  if (counter != nullptr && threadIdx.x == 0) *counter = 0u;
+}
+
+}  // namespace nt
+
 namespace {
 
 size_t image_bytes(int njobs, int nplans) {
   const size_t jb = sizeof(nt::BatchJob) * static_cast<size_t>(njobs);
   const size_t jb_al = (jb + 255) & ~static_cast<size_t>(255);
   return jb_al + sizeof(NtPlanK) * static_cast<size_t>(nplans);
+}
+
+int launch_batch(const void* dev_image, int njobs, int nplans, int64_t total_tiles, unsigned int* dev_counter,
+                 int grid, hipStream_t stream, const char* who) {
+  int g = grid;
+  if (g <= 0) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      hipDeviceProp_t prop;
+      if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    g = 4 * cus;   // several workgroups per CU keep loads and solves overlapping
+  }
+  if (g > total_tiles) g = static_cast<int>(total_tiles);
+  nt::BatchArgs a;
+  a.jobs = static_cast<const nt::BatchJob*>(dev_image);
+  const size_t jb = sizeof(nt::BatchJob) * static_cast<size_t>(njobs);
+  a.plans = reinterpret_cast<const NtPlanK*>(static_cast<const char*>(dev_image) +
+                                             ((jb + 255) & ~static_cast<size_t>(255)));
+  a.counter = dev_counter;
+  a.total_tiles = total_tiles;
+  a.njobs = njobs;
+  a.pad_ = 0;
+  hipLaunchKernelGGL(nt::batch_kernel, dim3(g), dim3(nt::BATCH_NT), 0, stream, a);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return nt::set_error(NT_EHIP, "%s: %s", who, hipGetErrorString(e));
+  return nt::set_error(NT_OK, "%s", "");
+}
+
+// Host staging of the allocation-free path's image and the per-device address
+// of `scratch`, both guarded by one mutex (calls from several host threads
+// serialize their packing and enqueueing).
+std::mutex g_noalloc_mu;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+unsigned char* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+alignas(256) unsigned char g_noalloc_host[NT_BATCH_NOALLOC_IMAGE_MAX];
+
+// The address of this device's `scratch`, looked up on its first use here with
+// that device current (caller holds g_noalloc_mu).
+int scratch_address(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return nt::set_error(NT_EHIP, "nt_batch_run_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return nt::set_error(NT_EUNSUPPORTED, "nt_batch_run_noalloc: device %d beyond %d", dev, NOALLOC_MAX_DEVICES);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return nt::set_error(NT_EHIP, "nt_batch_run_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return NT_OK;
 }
 
 }  // namespace
@@ -346,30 +428,45 @@ int nt_batch_launch(const void* dev_image, int njobs, int nplans, int64_t total_
   if (total_tiles >= 0xffffffffLL) return nt::set_error(NT_EINVAL, "%s", "nt_batch_launch: too many tiles");
   if (reinterpret_cast<uintptr_t>(dev_image) % 256 != 0)
     return nt::set_error(NT_EINVAL, "%s", "nt_batch_launch: image must be 256-byte aligned");
-  int g = grid;
-  if (g <= 0) {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-      hipDeviceProp_t prop;
-      if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-    }
-    g = 4 * cus;   // several workgroups per CU keep loads and solves overlapping
+  return launch_batch(dev_image, njobs, nplans, total_tiles, dev_counter, grid,
+                      static_cast<hipStream_t>(stream), "nt_batch_launch");
+}
+
+int nt_batch_run_noalloc(const nt_batch_job* jobs, int njobs, const nt_plan* const* plans, int nplans, int grid,
+                         void* stream) {
+  if (!jobs || !plans || njobs <= 0 || nplans <= 0)
+    return nt::set_error(NT_EINVAL, "%s", "nt_batch_run_noalloc: null or empty argument");
+  const size_t need = image_bytes(njobs, nplans);
+  if (need > NT_BATCH_NOALLOC_IMAGE_MAX)
+    return nt::set_error(NT_EINVAL,
+                         "nt_batch_run_noalloc: %d jobs and %d plans need a %zu-byte image; the static "
+                         "scratch holds at most %zu bytes (use nt_batch_pack + nt_batch_launch)",
+                         njobs, nplans, need, static_cast<size_t>(NT_BATCH_NOALLOC_IMAGE_MAX));
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  unsigned char* base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  int64_t total_tiles = 0;
+  if (int rc = nt_batch_pack(jobs, njobs, plans, nplans, g_noalloc_host, sizeof(g_noalloc_host), &total_tiles))
+    return rc;
+  if (total_tiles <= 0) return nt::set_error(NT_OK, "%s", "");
+  if (total_tiles >= 0xffffffffLL)
+    return nt::set_error(NT_EINVAL, "%s", "nt_batch_run_noalloc: too many tiles");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned int* counter = reinterpret_cast<unsigned int*>(base);
+  unsigned char* image = base + NOALLOC_HEAD;
+  // the image travels in the kernel arguments of fill launches, one chunk each;
+  // the first also zeroes the work counter (a kernel, not a memset)
+  for (size_t off = 0; off < need; off += nt::FILL_BYTES) {
+    const size_t len = (need - off < nt::FILL_BYTES) ? need - off : nt::FILL_BYTES;
+    nt::FillChunk c;
+    std::memcpy(c.w, g_noalloc_host + off, len);
+    hipLaunchKernelGGL(nt::scratch_fill_kernel, dim3(1), dim3(256), 0, s,
+                       reinterpret_cast<unsigned int*>(image + off), c, static_cast<int>(len / 4),
+                       off == 0 ? counter : nullptr);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return nt::set_error(NT_EHIP, "nt_batch_run_noalloc: fill: %s", hipGetErrorString(e));
   }
-  if (g > total_tiles) g = static_cast<int>(total_tiles);
-  nt::BatchArgs a;
-  a.jobs = static_cast<const nt::BatchJob*>(dev_image);
-  const size_t jb = sizeof(nt::BatchJob) * static_cast<size_t>(njobs);
-  a.plans = reinterpret_cast<const NtPlanK*>(static_cast<const char*>(dev_image) +
-                                             ((jb + 255) & ~static_cast<size_t>(255)));
-  a.counter = dev_counter;
-  a.total_tiles = total_tiles;
-  a.njobs = njobs;
-  a.pad_ = 0;
-  hipLaunchKernelGGL(nt::batch_kernel, dim3(g), dim3(nt::BATCH_NT), 0,
-                     static_cast<hipStream_t>(stream), a);
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return nt::set_error(NT_EHIP, "nt_batch_launch: %s", hipGetErrorString(e));
-  return nt::set_error(NT_OK, "%s", "");
+  return launch_batch(image, njobs, nplans, total_tiles, counter, grid, s, "nt_batch_run_noalloc");
 }
 
 }  // extern "C"

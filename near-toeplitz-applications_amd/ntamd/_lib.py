@@ -20,6 +20,7 @@ NT_F64 = 0
 NT_F32 = 1
 NT_PREC_F64 = 0
 NT_PREC_F32_REFINE = 1
+NT_BATCH_NOALLOC_IMAGE_MAX = 1048320   # bytes of job image the allocation-free batch holds
 
 # every symbol include/ntoeplitz.h declares, with its ctypes signature
 _i64 = ctypes.c_int64
@@ -61,6 +62,7 @@ SIGNATURES = {
     "nt_batch_image_bytes": (ctypes.c_size_t, [_int, _int]),
     "nt_batch_pack": (_int, [_vp, _int, _pp, _int, _vp, ctypes.c_size_t, _i64p]),
     "nt_batch_launch": (_int, [_vp, _int, _int, _i64, _vp, _int, _vp]),
+    "nt_batch_run_noalloc": (_int, [_vp, _int, _pp, _int, _int, _vp]),
 }
 
 

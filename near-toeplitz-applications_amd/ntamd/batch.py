@@ -18,7 +18,7 @@ import torch
 from . import _lib
 from .core import Plan, device_ptr, precision_code, stream_handle
 
-__all__ = ["NtBatchJob", "MixedBatch"]
+__all__ = ["NtBatchJob", "MixedBatch", "job_table", "run_noalloc"]
 
 COMPACT = (1.0, 2.0, 0.25, 1.0, 0.25, 2.0, 1.0)
 
@@ -32,6 +32,58 @@ class NtBatchJob(ctypes.Structure):
                 ("precision", ctypes.c_int), ("sweeps", ctypes.c_int)]
 
 
+def job_table(jobs):
+    """The ``nt_batch_job`` array, the distinct plans and the tensors to keep
+    alive for a list of job dicts (see ``MixedBatch``), validated."""
+    if not jobs:
+        raise ValueError("empty batch")
+    keep = []
+    plans, slot = [], {}
+    arr = (NtBatchJob * len(jobs))()
+    dev = None
+    for i, j in enumerate(jobs):
+        out = j["out"]
+        solve = bool(j.get("solve", False))
+        f = out if solve else j["f"]
+        fp = device_ptr(f, "f", dtypes=(torch.float64, torch.float32))
+        op = device_ptr(out, "out")
+        if f.dim() != 3 or tuple(out.shape) != tuple(f.shape):
+            raise _lib.NtInvalidArgument(_lib.NT_EINVAL, f"job {i}: f/out must be 3-D, same shape")
+        dev = out.device if dev is None else dev
+        axis = int(j["axis"])
+        n = int(f.shape[axis])
+        coeffs = tuple(float(c) for c in j.get("coeffs", COMPACT))
+        key = (n, coeffs)
+        if key not in slot:
+            slot[key] = len(plans)
+            plans.append(Plan(n, coeffs))
+        a = arr[i]
+        a.in_, a.out = fp.value, op.value
+        a.nz, a.ny, a.nx = (int(s) for s in f.shape)
+        a.dx = float(j.get("dx", 1.0))
+        a.axis = axis
+        a.producer = 0 if solve else 1
+        a.in_dtype = _lib.NT_F32 if f.dtype == torch.float32 else _lib.NT_F64
+        a.plan_slot = slot[key]
+        a.precision = precision_code(j.get("precision", "fp64"))
+        a.sweeps = int(j.get("sweeps", 1))
+        keep += [f, out]
+    return arr, plans, keep, dev
+
+
+def run_noalloc(jobs, grid=0):
+    """Run the jobs with ``nt_batch_run_noalloc`` on the current stream: the same
+    kernel as ``MixedBatch.run``, with the job image and work counter in the
+    library's static device array, so the call allocates no device or pinned
+    memory (for callers that cannot).  Calls overlapping on one GPU race on
+    that array; calls on one stream are ordered."""
+    lib = _lib.load()
+    arr, plans, keep, _ = job_table(jobs)
+    pv = (ctypes.c_void_p * len(plans))(*[p.handle.value for p in plans])
+    _lib.check(lib.nt_batch_run_noalloc(arr, len(jobs), pv, len(plans), int(grid), stream_handle()))
+    return keep
+
+
 class MixedBatch:
     """A fixed set of compact-derivative (or in-place solve) jobs run by one launch.
 
@@ -43,40 +95,8 @@ class MixedBatch:
     """
 
     def __init__(self, jobs, grid=0):
-        if not jobs:
-            raise ValueError("empty batch")
         lib = _lib.load()
-        self._keep = []
-        plans, slot = [], {}
-        arr = (NtBatchJob * len(jobs))()
-        dev = None
-        for i, j in enumerate(jobs):
-            out = j["out"]
-            solve = bool(j.get("solve", False))
-            f = out if solve else j["f"]
-            fp = device_ptr(f, "f", dtypes=(torch.float64, torch.float32))
-            op = device_ptr(out, "out")
-            if f.dim() != 3 or tuple(out.shape) != tuple(f.shape):
-                raise _lib.NtInvalidArgument(_lib.NT_EINVAL, f"job {i}: f/out must be 3-D, same shape")
-            dev = out.device if dev is None else dev
-            axis = int(j["axis"])
-            n = int(f.shape[axis])
-            coeffs = tuple(float(c) for c in j.get("coeffs", COMPACT))
-            key = (n, coeffs)
-            if key not in slot:
-                slot[key] = len(plans)
-                plans.append(Plan(n, coeffs))
-            a = arr[i]
-            a.in_, a.out = fp.value, op.value
-            a.nz, a.ny, a.nx = (int(s) for s in f.shape)
-            a.dx = float(j.get("dx", 1.0))
-            a.axis = axis
-            a.producer = 0 if solve else 1
-            a.in_dtype = _lib.NT_F32 if f.dtype == torch.float32 else _lib.NT_F64
-            a.plan_slot = slot[key]
-            a.precision = precision_code(j.get("precision", "fp64"))
-            a.sweeps = int(j.get("sweeps", 1))
-            self._keep += [f, out]
+        arr, plans, self._keep, dev = job_table(jobs)
         self.plans = plans
         pv = (ctypes.c_void_p * len(plans))(*[p.handle.value for p in plans])
         nbytes = lib.nt_batch_image_bytes(len(jobs), len(plans))

@@ -161,3 +161,30 @@ def test_tuning_table_set_get_and_validation():
     assert lib.nt_tune_set(1, 0, 64, 4) == _lib.NT_EINVAL
     assert b"variant" in lib.nt_last_error()
     assert lib.nt_tune_get(1, 0, 64, None, ctypes.byref(nv)) == 0 and nv.value == 4
+
+
+def test_noalloc_batch_limits_without_a_gpu():
+    """nt_batch_run_noalloc: the image limit of the header is the one the Python
+    layer states, and empty or oversized batches are refused before any device
+    call (so this runs without a GPU)."""
+    from ntamd import _lib
+    from ntamd.batch import NtBatchJob
+    lib = _lib.load()
+    text = open(HEADER).read()
+    m = re.search(r"#define NT_BATCH_NOALLOC_IMAGE_MAX \(\(size_t\)(\d+)\)", text)
+    assert m and int(m.group(1)) == _lib.NT_BATCH_NOALLOC_IMAGE_MAX
+    assert _lib.NT_BATCH_NOALLOC_IMAGE_MAX + 256 == 1 << 20
+    h = ctypes.c_void_p()
+    co = (ctypes.c_double * 7)(1, 2, .25, 1, .25, 2, 1)
+    assert lib.nt_plan_create(ctypes.c_int64(64), co, ctypes.byref(h)) == 0
+    pv = (ctypes.c_void_p * 1)(h.value)
+    try:
+        assert lib.nt_batch_run_noalloc((NtBatchJob * 1)(), 0, pv, 1, 0, None) == _lib.NT_EINVAL
+        njobs = 1
+        while lib.nt_batch_image_bytes(njobs, 1) <= _lib.NT_BATCH_NOALLOC_IMAGE_MAX:
+            njobs += 1000
+        arr = (NtBatchJob * njobs)()
+        assert lib.nt_batch_run_noalloc(arr, njobs, pv, 1, 0, None) == _lib.NT_EINVAL
+        assert b"static scratch" in lib.nt_last_error()
+    finally:
+        lib.nt_plan_destroy(h)
