@@ -330,6 +330,14 @@ int nt_compact_derivative(const nt_plan* plan, const void* f, int f_dtype, void*
   if (!(std::isfinite(dx) && dx != 0.0)) return fail(NT_EINVAL, "nt_compact_derivative: dx=%g", dx);
   if (int rc = check_dims(nz, ny, nx)) return rc;
   if (int rc = check_precision(precision, sweeps)) return rc;
+  {
+    // the kernels read one tile while storing another, so any shared byte of f
+    // and out (not only equal base pointers) would corrupt the result
+    const uintptr_t npts = static_cast<uintptr_t>(nz) * static_cast<uintptr_t>(ny) * static_cast<uintptr_t>(nx);
+    const uintptr_t f0 = reinterpret_cast<uintptr_t>(f), o0 = reinterpret_cast<uintptr_t>(out);
+    const uintptr_t f1 = f0 + npts * (f_dtype == NT_F64 ? 8u : 4u), o1 = o0 + npts * 8u;
+    if (f0 < o1 && o0 < f1) return fail(NT_EINVAL, "nt_compact_derivative: f and out overlap");
+  }
   const AxisGeom g = axis_geom(nz, ny, nx, axis);
   if (g.n != plan->n)
     return fail(NT_EINVAL, "nt_compact_derivative: line length %lld != plan n %lld",

@@ -188,3 +188,23 @@ def test_noalloc_batch_limits_without_a_gpu():
         assert b"static scratch" in lib.nt_last_error()
     finally:
         lib.nt_plan_destroy(h)
+
+
+def test_compact_derivative_refuses_overlapping_buffers():
+    """f and out sharing any byte (not only the same base address) are refused
+    before a launch: the tile kernels read one tile while storing another."""
+    from ntamd import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    co = (ctypes.c_double * 7)(1, 2, .25, 1, .25, 2, 1)
+    assert lib.nt_plan_create(ctypes.c_int64(64), co, ctypes.byref(h)) == 0
+    try:
+        base = 1 << 32                      # addresses only: nothing is dereferenced
+        npts = 4 * 4 * 64
+        for f_dtype, esize in ((_lib.NT_F64, 8), (_lib.NT_F32, 4)):
+            for off in (8, npts * esize - 8, -npts * 8 + 8):
+                rc = lib.nt_compact_derivative(h, ctypes.c_void_p(base), f_dtype, ctypes.c_void_p(base + off),
+                                               _lib.NT_F64, 4, 4, 64, 2, 0.1, 0, 1, None)
+                assert rc == _lib.NT_EINVAL and b"overlap" in lib.nt_last_error(), (f_dtype, off)
+    finally:
+        lib.nt_plan_destroy(h)

@@ -172,9 +172,11 @@ def test_1024cubed_fp32_storage_sampled_lines(dev, field_1024_fp32, axis, precis
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
 @pytest.mark.parametrize("shape", [(1024, 1024, 768), (1024, 999, 801)])
 def test_z_lines_rows_6mib_apart(dev, shape, precision):
-    """FP64 z-lines with rows at least 6 MiB apart, which the general tile kernel
-    walks in contiguous ranges: 786,432 lines in whole tiles and 800,199 lines
-    with a ragged last tile.  256 lines drawn at random equal the oracle."""
+    """FP64-stored z-lines with rows at least 6 MiB apart, walked in contiguous
+    tile ranges (the FP64 solve on the general tile kernel; FP32 + refinement
+    takes the whole-tile fast path for whole tiles): 786,432 lines in whole
+    tiles and 800,199 lines with a ragged last tile.  256 lines drawn at random
+    equal the oracle."""
     from ntamd import COMPACT_COEFFS, Plan, core
     from ntamd.workloads import compact_field_device
     n = shape[0]
