@@ -3,20 +3,29 @@
 //
 // The host packs a binary image (job table + level tables of the plans the
 // jobs use) into a caller-provided host buffer; the caller copies it to the
-// device and zeroes a work counter; one persistent launch then pulls tiles of
-// 16 lines from the counter until every job is done.  A tile is 16 lines
-// adjacent in memory (16 rows of an x-line field, or 16 adjacent y/z lines),
-// staged through LDS with all of a thread's loads in flight at once; each of
-// the 8 wavefronts then solves two of the lines with the same register CR as
-// the other kernels, 64 lanes x R rows with R = 1, 2, 4 or 8 by line length
-// (up to 512 points).
-// Per-line arithmetic does not depend on which other jobs share the launch, so
-// a field gives the same bits alone and in any mix.
+// device and zeroes a work counter; one persistent launch then pulls chunks
+// (consecutive 16-line tiles of one job, about 128 K points) from the counter
+// until every job is done.
+//
+// Each job has a class: layout (contiguous x-lines or strided y/z-lines),
+// rows per lane R = 1, 2, 4 or 8 (64 lanes x R >= the line length, up to 512
+// points), whether the line length is exactly 64 R (a power of two, so the
+// level structure is known at compile time), precision, and producer with its
+// storage type.  Every class has its own compile-time specialised loop, with
+// the loop shapes of the per-shape kernels: contiguous lines are worked
+// through by each wavefront on its own (next line prefetched into registers,
+// 16-byte coalesced accesses, LDS transpose for R >= 4); strided lines go
+// through whole 16-line tiles in LDS, two lines per wavefront, with the next
+// tile prefetched into registers during the solve.  The level factors are
+// read with scalar loads.
+// Per-line arithmetic depends only on the job, so a field gives the same bits
+// alone and in any mix.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/ntoeplitz.h"
 #include "nt_error.h"
@@ -28,16 +37,34 @@ namespace nt {
 constexpr int BATCH_MAXN = 512;
 constexpr int BATCH_TS = 16;                // lines per tile
 constexpr int BATCH_WAVES = 8;              // wavefronts per workgroup
-constexpr int BATCH_LPW = BATCH_TS / BATCH_WAVES;   // lines solved by each wavefront
 constexpr int BATCH_NT = BATCH_WAVES * 64;
-// LDS row of a line: point e of a line with R = 2^rsel rows per lane sits at
-// e + (e >> rsel), i.e. one double of padding after each lane's chunk, so a
-// lane's chunk starts R + 1 doubles after its neighbour's (an odd number of
-// 8-byte bank pairs for R >= 2): the solve's chunk reads and writes do not
-// collide.  A row then holds at most 64 (R + 1) doubles.
-constexpr int BATCH_EP = BATCH_MAXN + BATCH_MAXN / 8 + 2;   // LDS stride between lines (doubles)
-// every thread moves BATCH_TS * BATCH_MAXN / BATCH_NT elements of a tile
-constexpr int BATCH_PER = BATCH_TS * BATCH_MAXN / BATCH_NT;
+// A chunk is the unit handed out by the work counter: consecutive tiles of one
+// job (at least one).  Its size in points is the batch's total divided by
+// BATCH_CHUNKS_PER_SLOT x 512 (the resident workgroups of a full MI355X),
+// kept within [BATCH_CHUNK_MIN_PTS, BATCH_CHUNK_MAX_PTS]: large enough that a
+// chunk's fixed cost (finding its job, the first tile's loads) is small, small
+// enough that the last chunks do not leave most of the GPU idle.
+constexpr int64_t BATCH_CHUNK_MAX_PTS = int64_t(1) << 17;
+constexpr int64_t BATCH_CHUNK_MIN_PTS = int64_t(1) << 13;
+constexpr int64_t BATCH_CHUNKS_PER_SLOT = 8;
+
+// LDS doubles per staged line of a strided tile (two lines per wavefront):
+// lane l's chunk of R points starts R + 1 doubles after lane l-1's, an odd
+// number of 8-byte bank pairs, so the solve's chunk reads do not collide.
+__host__ __device__ constexpr int batch_ep(int R) { return 64 * (R + (R >= 2 ? 1 : 0)) + 1; }
+// LDS doubles per contiguous line (one per wavefront): chunks R + 2 apart, so
+// every chunk starts 16-byte aligned.
+__host__ __device__ constexpr int batch_ls(int R) { return 64 * (R + 2); }
+constexpr int BATCH_LDS = (BATCH_TS * batch_ep(8) > BATCH_WAVES * batch_ls(8)) ? BATCH_TS * batch_ep(8)
+                                                                               : BATCH_WAVES * batch_ls(8);
+
+// What a job computes and from which storage type.
+enum BatchKind : int { K_SOLVE = 0, K_CD64 = 1, K_CD32 = 2 };
+// Class of a job: every combination has a compile-time specialised loop.
+constexpr int NUM_CLASSES = 2 * 4 * 2 * 2 * 3;
+__host__ __device__ constexpr int batch_class(int layout, int rsel, int full, int prec, int kind) {
+  return (((layout * 4 + rsel) * 2 + full) * 2 + prec) * 3 + kind;
+}
 
 struct BatchJob {
   const void* in;
@@ -45,85 +72,127 @@ struct BatchJob {
   int64_t pstride, estride, ostride;  // line geometry (as LineArgs)
   int64_t phys;                       // lines per slab
   int64_t tiles;                      // tiles per slab
-  int64_t tile0;                      // first global tile of this job
   int64_t ntiles;                     // tiles of this job (slabs x tiles)
+  int64_t chunk0;                     // first global chunk of this job
+  int64_t chunk_tiles;                // tiles per chunk
   double s1, s2;                      // compact scalars 3/(4h), 1/(2h)
   int32_t plan;                       // index into the plan table
-  int32_t layout;                     // CONTIG / STRIDED
-  int32_t prod;                       // P_IDENT / P_COMPACT
-  int32_t in_f32;                     // input stored as FP32
-  int32_t prec;                       // PREC_F64 / PREC_F32R
+  int32_t cls;                        // batch_class(...)
   int32_t sweeps;
+  int32_t vec;                        // buffers and strides allow 16-byte accesses
 };
 
 struct BatchArgs {
   const BatchJob* jobs;
   const NtPlanK* plans;
   unsigned int* counter;
-  int64_t total_tiles;
+  int64_t total;                      // chunks over all jobs
   int32_t njobs;
   int32_t pad_;
 };
 
-// Element k of this thread's share of a tile: (line, position) in the tile.
-// Contiguous lines: wavefront w moves lines w, w + WAVES, ...; lane l takes
-// positions l, l + 64, ... (each wavefront reads whole 512 B row runs).
-// Strided lines: the TS lines are adjacent in memory, so TS consecutive
-// threads read one TS * 8 B run of a row; thread t takes line t % TS,
-// positions t / TS + k * (NT / TS).
-__device__ __forceinline__ void tile_slot(int layout, int tid, int k, int& line, int& e) {
-  if (layout == CONTIG) {
-    constexpr int PER_LINE = BATCH_MAXN / 64;        // positions per lane per line
-    line = (tid >> 6) + (k / PER_LINE) * BATCH_WAVES;
-    e = (tid & 63) + (k % PER_LINE) * 64;
-  } else {
-    line = tid % BATCH_TS;
-    e = tid / BATCH_TS + k * (BATCH_NT / BATCH_TS);
-  }
-}
+// The job and plan tables are read through the constant address space, i.e.
+// with scalar loads: written before the launch, only read by the kernel.  As
+// vector loads they would wait, in order, behind the previous tile's stores;
+// and the level factors stay scalar operands of the solve.
+using CJob = const __attribute__((address_space(4))) BatchJob;
+using CPlan = const __attribute__((address_space(4))) NtPlanK;
 
-template <typename TIn>
-__device__ __forceinline__ void stage_tile(double* lds, const TIn* __restrict__ in, int layout, int tid, int rows,
-                                           int n, int rsel, int64_t base, int64_t pstride, int64_t estride) {
-  double v[BATCH_PER];
-  // issue every load of the tile before the first LDS write so they overlap
-#pragma unroll
-  for (int k = 0; k < BATCH_PER; ++k) {
-    int line, e;
-    tile_slot(layout, tid, k, line, e);
-    v[k] = (line < rows && e < n)
-               ? static_cast<double>(__builtin_nontemporal_load(in + base + line * pstride + e * estride))
-               : 0.0;
-  }
-#pragma unroll
-  for (int k = 0; k < BATCH_PER; ++k) {
-    int line, e;
-    tile_slot(layout, tid, k, line, e);
-    if (line < rows && e < n) lds[line * BATCH_EP + e + (e >> rsel)] = v[k];
-  }
-}
+template <int KIND>
+using BatchIn = typename std::conditional<KIND == K_CD32, float, double>::type;
 
-// One line held in LDS row `row` (n points, padded as described at BATCH_EP):
-// lane l takes points l R .. l R + R - 1, builds the right-hand side (compact
-// producer) or takes the values as they are (solve), runs the register CR and
-// writes the solution back over the row.
-template <int R, bool FULL>
-__device__ __forceinline__ void solve_line(double* row, int n, int prod, int prec, const NtPlanK& P, double s1,
-                                           double s2, int sweeps, int lane) {
-  constexpr int RS = (R == 1) ? 0 : (R == 2) ? 1 : (R == 4) ? 2 : 3;
-  const int my_e0 = lane * R;
-  double* chunk = row + lane * (R + 1);
-  double f[R];
+// ---- contiguous lines (x-lines): every wavefront works through its own lines ----
+// Wavefront w takes lines q0 + w, q0 + w + 8, ... of the chunk.  The next
+// line's loads are issued into registers before the current line is solved.
+// Up to 2 rows per lane a lane loads its own R consecutive points directly
+// (64 x 8 R contiguous bytes per wavefront); longer lines are loaded with
+// coalesced 16-byte accesses into the wavefront's LDS rows and transposed into
+// chunks there.  Loads are unconditional (indexes clamped into the line), so
+// the compiler can count them.
+template <typename TIn, int R, bool FULL>
+__device__ __forceinline__ void cline_fetch(const TIn* __restrict__ src, int n, int lane, bool vec, TIn (&pf)[R]) {
+  if (R <= 2) {
+    if (R == 2 && vec) {                       // a lane's two points as one access
+      const int j = FULL ? lane : min(lane, n / 2 - 1);
+      if constexpr (sizeof(TIn) == 8) {
+        const nt_dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(src) + j);
+        pf[0] = v.x;
+        pf[R - 1] = v.y;
+      } else {
+        typedef float nt_flt2 __attribute__((ext_vector_type(2)));
+        const nt_flt2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_flt2*>(src) + j);
+        pf[0] = v.x;
+        pf[R - 1] = v.y;
+      }
+    } else {
 #pragma unroll
-  for (int k = 0; k < R; ++k) f[k] = (FULL || my_e0 + k < n) ? chunk[k] : 0.0;
-  double d[R];
-  if (prod == P_COMPACT) {
-    double bnd[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const int e = (i < 3) ? i : n + 2 - i;
-      bnd[i] = row[e + (e >> RS)];
+      for (int k = 0; k < R; ++k) {
+        const int e = lane * R + k;
+        pf[k] = __builtin_nontemporal_load(src + (FULL ? e : min(e, n - 1)));
+      }
     }
+  } else if (vec) {
+    constexpr int V = 16 / static_cast<int>(sizeof(TIn));
+    const int nvec = n / V;
+#pragma unroll
+    for (int i = 0; i < R / V; ++i) {
+      const int j = FULL ? lane + 64 * i : min(lane + 64 * i, nvec - 1);
+      if constexpr (sizeof(TIn) == 8) {
+        const nt_dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(src) + j);
+        pf[2 * i] = v.x;
+        pf[2 * i + 1] = v.y;
+      } else {
+        const nt_flt4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_flt4*>(src) + j);
+        pf[4 * i] = v.x;
+        pf[4 * i + 1] = v.y;
+        pf[4 * i + 2] = v.z;
+        pf[4 * i + 3] = v.w;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int e = lane + 64 * i;
+      pf[i] = __builtin_nontemporal_load(src + (FULL ? e : min(e, n - 1)));
+    }
+  }
+}
+
+// LDS position of point e of a contiguous line (chunks R + 2 apart)
+template <int R>
+__device__ __forceinline__ int cl_at(int e) {
+  return (e / R) * (R + 2) + (e % R);
+}
+
+// the fetched values into the wavefront's LDS rows (R >= 4); clamped indexes
+// write a point's own value a second time
+template <typename TIn, int R, bool FULL>
+__device__ __forceinline__ void cline_deposit(double* __restrict__ L, int n, int lane, bool vec, const TIn (&pf)[R]) {
+  if (vec) {
+    constexpr int V = 16 / static_cast<int>(sizeof(TIn));
+    const int nvec = n / V;
+#pragma unroll
+    for (int i = 0; i < R / V; ++i) {
+      const int j = FULL ? lane + 64 * i : min(lane + 64 * i, nvec - 1);
+#pragma unroll
+      for (int c = 0; c < V; ++c) L[cl_at<R>(V * j + c)] = static_cast<double>(pf[V * i + c]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int e = lane + 64 * i;
+      L[cl_at<R>(FULL ? e : min(e, n - 1))] = static_cast<double>(pf[i]);
+    }
+  }
+}
+
+// right-hand side (compact) or the values themselves (solve), then the solve
+template <int R, bool FULL, int PREC, int KIND>
+__device__ __forceinline__ void batch_solve(const double (&f)[R], const double (&bnd)[6], int my_e0, int n,
+                                            double s1, double s2, const NtPlanK& P, int lane, int sweeps,
+                                            double (&x)[R]) {
+  double d[R];
+  if constexpr (KIND != K_SOLVE) {
     const double fl = shup<64>(f[R - 1], 1);
     const double fr = shdn<64>(f[0], 1);
     compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, s1, s2, d);
@@ -131,124 +200,381 @@ __device__ __forceinline__ void solve_line(double* row, int n, int prod, int pre
 #pragma unroll
     for (int k = 0; k < R; ++k) d[k] = f[k];
   }
-  double x[R];
-  if (prec == PREC_F64) solve_chunk<R, 64, PREC_F64, FULL>(d, x, P, lane, sweeps);
-  else solve_chunk<R, 64, PREC_F32R, FULL>(d, x, P, lane, sweeps);
-#pragma unroll
-  for (int k = 0; k < R; ++k)
-    if (FULL || my_e0 + k < n) chunk[k] = x[k];
+  solve_chunk<R, 64, PREC, FULL>(d, x, P, lane, sweeps);
 }
 
-__global__ void __launch_bounds__(BATCH_NT) batch_kernel(const BatchArgs args) {
-  __shared__ __attribute__((aligned(16))) double lds[BATCH_TS * BATCH_EP];
-  __shared__ __attribute__((aligned(16))) NtPlanK s_plan;   // level table of the current job
-  __shared__ int64_t s_tile[2];     // tile indices, alternating slots
-  // reset through an opaque move at every tile, so the per-thread offsets
-  // derived from it are recomputed per tile, not hoisted and held across the loop
-  int tid = threadIdx.x;
+template <int R, bool FULL, int PREC, int KIND>
+__device__ __forceinline__ void contig_chunk(CJob& J, const NtPlanK& P, int64_t t0, int64_t t1, double* lds) {
+  using TIn = BatchIn<KIND>;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  // The job and plan tables are read through the constant address space, i.e.
-  // with scalar loads: as vector loads they would wait, in order, for the
-  // previous tile's stores before the next tile could be located and staged.
-  // The tables are written before the launch and only read by the kernel.
-  using CJob = const __attribute__((address_space(4))) BatchJob;
-  using CPlan = const __attribute__((address_space(4))) NtPlanK;
-  CJob* jobs = (CJob*)(args.jobs);
-  CPlan* plans = (CPlan*)(args.plans);
-  int j = 0;                        // current job: tiles come in increasing order
-  int plan_in_lds = -1;
-  int slot = 0;
+  const int n = FULL ? 64 * R : P.n;
+  const int64_t phys = J.phys;
+  const int64_t pstride = J.pstride;
+  const int64_t q_end = (t1 * BATCH_TS < phys) ? t1 * BATCH_TS : phys;
+  int64_t q = t0 * BATCH_TS + wave;
+  if (q >= q_end) return;                       // uniform across the wavefront
+  const TIn* __restrict__ in = static_cast<const TIn*>(J.in);
+  double* __restrict__ out = static_cast<double*>(J.out);
+  const bool vec = J.vec != 0;
+  const double s1 = J.s1, s2 = J.s2;
+  const int sweeps = J.sweeps;
+  const int my_e0 = lane * R;
+  double* __restrict__ L = lds + wave * batch_ls(R);
+  double* __restrict__ mine = L + lane * (R + 2);
 
-  // One workgroup barrier per tile, at the top: every wavefront has finished the
-  // previous tile (its stores have read the LDS, its solve is done with s_plan),
-  // and the new tile index is visible.  The index alternates between two slots,
-  // so writing the next one cannot race with late readers of this one.  Strided
-  // tiles spread each line over all threads and add two barriers; contiguous
-  // tiles are staged, solved and stored by each wavefront on its own lines.
+  TIn pf[R];
+  // an opaque lane index here too: offsets derived from threadIdx.x alone
+  // would be hoisted out of the loop over chunks
+  cline_fetch<TIn, R, FULL>(in + q * pstride, n, (static_cast<int>(threadIdx.x) + opaque_vzero()) & 63, vec, pf);
   for (;;) {
-    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (BATCH_NT - 1);   // the mask keeps its range known
-    if (tid == 0) s_tile[slot] = static_cast<int64_t>(atomicAdd(args.counter, 1u));
-    __syncthreads();
-    // the tile index is the same for the whole workgroup: keep it (and the
-    // job index derived from it) in scalar registers
-    const int64_t t = static_cast<int64_t>(static_cast<unsigned>(uni(static_cast<int>(s_tile[slot]))));
-    slot ^= 1;
-    if (t >= args.total_tiles) break;            // every workgroup reaches this
-    while (j + 1 < args.njobs && jobs[j + 1].tile0 <= t) ++j;
-    const CJob& J = jobs[j];
-    const int plan = J.plan;
-    if (plan != plan_in_lds) {                   // copy the level table into LDS (uniform branch)
-      const int* src = reinterpret_cast<const int*>(args.plans + plan);
-      int* dst = reinterpret_cast<int*>(&s_plan);
-      for (int i = tid; i < static_cast<int>(sizeof(NtPlanK) / 4); i += BATCH_NT) dst[i] = src[i];
-      plan_in_lds = plan;
-      __syncthreads();
-    }
-    const int64_t local = t - J.tile0;
-    const int64_t o = local / J.tiles;
-    const int64_t p0 = (local - o * J.tiles) * BATCH_TS;
-    const int64_t base = o * J.ostride + p0 * J.pstride;
-    const int64_t rows64 = J.phys - p0;
-    const int rows = rows64 < BATCH_TS ? static_cast<int>(rows64) : BATCH_TS;
-    const int layout = J.layout, prod = J.prod, prec = J.prec;
-    const int n = plans[plan].n;
-    const int64_t pstride = J.pstride, estride = J.estride;
-    double* __restrict__ out = static_cast<double*>(J.out);
-
-    // rows per lane R = 2^rsel follow the line length (64 lanes x R >= n)
-    const int rsel = (n <= 64) ? 0 : (n <= 128) ? 1 : (n <= 256) ? 2 : 3;
-
-    // ---- stage the tile (TS lines x n) through LDS, coalesced along memory ------
-    if (J.in_f32)
-      stage_tile(lds, static_cast<const float*>(J.in), layout, tid, rows, n, rsel, base, pstride, estride);
-    else
-      stage_tile(lds, static_cast<const double*>(J.in), layout, tid, rows, n, rsel, base, pstride, estride);
-    if (layout == CONTIG) wave_lds_sync();       // uniform: a wavefront staged its own lines
-    else __syncthreads();
-
-    // ---- each wavefront solves its lines; a line's LDS row is read and then
-    // overwritten by the same wavefront only, so no barrier in between.  The
-    // compile-time level structure applies when n == 64 R. -----------------------
-    const NtPlanK& P = s_plan;
-    const bool full = (n == (64 << rsel));
-    // the job's scalars in scalar registers: loaded inside the solve they would
-    // be vector loads, and each would wait for the previous tile's stores (the
-    // vector-memory counter is in order)
-    const double s1 = J.s1, s2 = J.s2;
-    const int sweeps = J.sweeps;
-#pragma unroll 1
-    for (int h = 0; h < BATCH_LPW; ++h) {
-      const int ql = wave + h * BATCH_WAVES;
-      if (ql >= rows) break;                     // uniform across the wavefront
-      double* row = lds + ql * BATCH_EP;
-      if (full) {
-        switch (rsel) {
-          case 0: solve_line<1, true>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
-          case 1: solve_line<2, true>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
-          case 2: solve_line<4, true>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
-          default: solve_line<8, true>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
+    // the lane index through an opaque move per line: the offsets derived from
+    // it are recomputed, not held across the loop
+    const int ln = (static_cast<int>(threadIdx.x) + opaque_vzero()) & 63;
+    const int64_t qn = q + BATCH_WAVES;
+    const bool more = qn < q_end;               // uniform
+    double f[R];
+    double bnd[6];
+    if constexpr (R <= 2) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) f[k] = (FULL || my_e0 + k < n) ? static_cast<double>(pf[k]) : 0.0;
+      if constexpr (KIND != K_SOLVE) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {           // point e sits in lane e / R, register e % R
+          const int e = (i < 3) ? i : n + 2 - i;
+          const double v = (R == 1 || (e & 1) == 0) ? f[0] : f[R - 1];
+          bnd[i] = shidx<64>(v, e / R);
+        }
+      }
+    } else {
+      wave_lds_sync();                          // the previous line's stores have read the rows
+      cline_deposit<TIn, R, FULL>(L, n, ln, vec, pf);
+      wave_lds_sync();
+      if (FULL || my_e0 + R <= n) {
+#pragma unroll
+        for (int k = 0; k < R; k += 2) {
+          const nt_dbl2 t = *reinterpret_cast<const nt_dbl2*>(mine + k);
+          f[k] = t.x;
+          f[k + 1] = t.y;
         }
       } else {
-        switch (rsel) {
-          case 0: solve_line<1, false>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
-          case 1: solve_line<2, false>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
-          case 2: solve_line<4, false>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
-          default: solve_line<8, false>(row, n, prod, prec, P, s1, s2, sweeps, lane); break;
+#pragma unroll
+        for (int k = 0; k < R; ++k) f[k] = (my_e0 + k < n) ? mine[k] : 0.0;
+      }
+      if constexpr (KIND != K_SOLVE) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bnd[i] = L[cl_at<R>((i < 3) ? i : n + 2 - i)];
+      }
+    }
+    if (more) cline_fetch<TIn, R, FULL>(in + qn * pstride, n, ln, vec, pf);     // in flight during the solve
+    double x[R];
+    batch_solve<R, FULL, PREC, KIND>(f, bnd, my_e0, n, s1, s2, P, lane, sweeps, x);
+    double* __restrict__ dst = out + q * pstride;
+    if constexpr (R <= 2) {
+      if (R == 2 && vec && (FULL || my_e0 + 2 <= n)) {
+        nt_dbl2 t;
+        t.x = x[0];
+        t.y = x[R - 1];
+        __builtin_nontemporal_store(t, reinterpret_cast<nt_dbl2*>(dst) + lane);
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (FULL || my_e0 + k < n) __builtin_nontemporal_store(x[k], dst + my_e0 + k);
+      }
+    } else {
+      wave_lds_sync();                          // every lane has read its chunk and the closures
+      if (FULL || my_e0 + R <= n) {
+#pragma unroll
+        for (int k = 0; k < R; k += 2) {
+          nt_dbl2 t;
+          t.x = x[k];
+          t.y = x[k + 1];
+          *reinterpret_cast<nt_dbl2*>(mine + k) = t;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (my_e0 + k < n) mine[k] = x[k];
+      }
+      wave_lds_sync();
+      const int ls = (static_cast<int>(threadIdx.x) + opaque_vzero()) & 63;   // fresh, as ln
+      if (vec) {
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i) {
+          const int j = ls + 64 * i;
+          if (FULL || j < n / 2) {
+            nt_dbl2 t;
+            t.x = L[cl_at<R>(2 * j)];
+            t.y = L[cl_at<R>(2 * j + 1)];
+            __builtin_nontemporal_store(t, reinterpret_cast<nt_dbl2*>(dst) + j);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const int e = ls + 64 * i;
+          if (FULL || e < n) __builtin_nontemporal_store(L[cl_at<R>(e)], dst + e);
         }
       }
     }
-    if (layout == CONTIG) wave_lds_sync();
-    else __syncthreads();
+    if (!more) break;
+    q = qn;
+  }
+}
 
-    // ---- store ----------------------------------------------------------------------
+// ---- strided lines (y- and z-lines): 16-line whole tiles in LDS ------------------------
+// A tile is 16 adjacent lines, whose rows are runs of 16 consecutive elements
+// in memory; the whole tile sits in LDS and each of the 8 wavefronts solves
+// two of its lines.  The next tile's values are loaded into registers while
+// the current tile is solved and stored.
+template <int R, bool FULL, int PREC, int KIND>
+__device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t t0, int64_t t1, double* lds) {
+  using TIn = BatchIn<KIND>;
+  constexpr int CP = (R >= 2) ? 1 : 0;
+  constexpr int EP = batch_ep(R);
+  constexpr int NMAX = 64 * R;
+  constexpr int VIN = 16 / static_cast<int>(sizeof(TIn));
+  constexpr int VPR = BATCH_TS / VIN;                          // input vectors per row
+  constexpr int PFV = (VPR * NMAX + BATCH_NT - 1) / BATCH_NT;  // input vectors per thread
+  constexpr int PFS = BATCH_TS * NMAX / BATCH_NT;              // scalars per thread (2 R)
+  constexpr int PF = (PFV * VIN > PFS) ? PFV * VIN : PFS;
+  // A thread's fetch vectors reach past the tile's last point when PFV
+  // threads-worth exceed the tile (FP32 rows at R = 1): clamp there too.
+  constexpr bool CLAMP_V = !FULL || (PFV * BATCH_NT > VPR * NMAX);
+  constexpr int VPO = BATCH_TS / 2;                            // output vectors per row
+  constexpr int PFO = VPO * NMAX / BATCH_NT;                   // output vectors per thread (R)
+  const int n = FULL ? NMAX : P.n;
+  const int64_t estride = J.estride, ostride = J.ostride, phys = J.phys, tiles = J.tiles;
+  const bool jvec = J.vec != 0;
+  const double s1 = J.s1, s2 = J.s2;
+  const int sweeps = J.sweeps;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int my_e0 = lane * R;
+  // Reset at the top of every tile through an opaque move, so the per-thread
+  // load, store and LDS offsets derived from it are recomputed for each tile
+  // instead of being hoisted out of the tile loop (or, for the chunk's first
+  // fetch, out of the loop over chunks) and held across it.
+  int tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (BATCH_NT - 1);
+  const TIn* __restrict__ in = static_cast<const TIn*>(J.in);
+  double* __restrict__ out = static_cast<double*>(J.out);
+  auto at = [&](int row, int el) -> int { return row * EP + el + (el / R) * CP; };
+
+  struct Tile {
+    int64_t base;
+    int32_t rows;
+    int32_t vec;
+  };
+  auto geometry = [&](int64_t t) -> Tile {
+    Tile g;
+    const int64_t o = t / tiles;
+    const int64_t p0 = (t - o * tiles) * BATCH_TS;
+    g.base = o * ostride + p0;
+    const int64_t rv = phys - p0;
+    g.rows = rv < BATCH_TS ? static_cast<int>(rv) : BATCH_TS;
+    g.vec = (jvec && g.rows == BATCH_TS) ? 1 : 0;
+    return g;
+  };
+
+  TIn pf[PF];
+  auto fetch = [&](const Tile& g, int i0, int i1) {     // vectors i0 .. i1-1 (the scalar path: all)
+    const TIn* __restrict__ src = in + g.base;
+    if (g.vec) {
 #pragma unroll
-    for (int k = 0; k < BATCH_PER; ++k) {
-      int line, e;
-      tile_slot(layout, tid, k, line, e);
-      if (line < rows && e < n)
-        __builtin_nontemporal_store(lds[line * BATCH_EP + e + (e >> rsel)], out + base + line * pstride + e * estride);
+      for (int i = i0; i < i1; ++i) {
+        const int idx = tid + i * BATCH_NT;
+        const int vr = idx % VPR;
+        const int el = CLAMP_V ? min(idx / VPR, n - 1) : idx / VPR;
+        const TIn* p = src + static_cast<int64_t>(el) * estride + vr * VIN;
+        if constexpr (sizeof(TIn) == 8) {
+          const nt_dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(p));
+          pf[2 * i] = v.x;
+          pf[2 * i + 1] = v.y;
+        } else {
+          const nt_flt4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_flt4*>(p));
+          pf[4 * i] = v.x;
+          pf[4 * i + 1] = v.y;
+          pf[4 * i + 2] = v.z;
+          pf[4 * i + 3] = v.w;
+        }
+      }
+    } else if (i0 == 0) {
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * BATCH_NT;
+        const int r = idx % BATCH_TS, el = idx / BATCH_TS;
+        if (el < n && r < g.rows) pf[i] = src[r + static_cast<int64_t>(el) * estride];
+      }
     }
+  };
+  // Vectors of the next tile fetched before the solve (all of them); the rest
+  // would follow it.
+  constexpr int PF_EARLY = PFV;
+  auto deposit = [&](const Tile& g) {
+    if (g.vec) {
+#pragma unroll
+      for (int i = 0; i < PFV; ++i) {
+        const int idx = tid + i * BATCH_NT;
+        const int vr = idx % VPR;
+        const int el = CLAMP_V ? min(idx / VPR, n - 1) : idx / VPR;
+#pragma unroll
+        for (int c = 0; c < VIN; ++c) lds[at(vr * VIN + c, el)] = static_cast<double>(pf[VIN * i + c]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * BATCH_NT;
+        const int r = idx % BATCH_TS, el = idx / BATCH_TS;
+        if (el < n && r < g.rows) lds[at(r, el)] = static_cast<double>(pf[i]);
+      }
+    }
+  };
+  auto solve_tile = [&](const Tile& g) {
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      const int ql = wave + h * BATCH_WAVES;     // tile-local line of this wavefront
+      const bool active = ql < g.rows;           // uniform across the wavefront
+      double x[R];
+      if (active) {
+        double f[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) f[k] = (FULL || my_e0 + k < n) ? lds[at(ql, my_e0 + k)] : 0.0;
+        double bnd[6];
+        if constexpr (KIND != K_SOLVE) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) bnd[i] = lds[at(ql, (i < 3) ? i : n + 2 - i)];
+        }
+        batch_solve<R, FULL, PREC, KIND>(f, bnd, my_e0, n, s1, s2, P, lane, sweeps, x);
+      }
+      wave_lds_sync();                           // the solution overwrites the line just read
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (FULL || my_e0 + k < n) lds[at(ql, my_e0 + k)] = x[k];
+      }
+    }
+  };
+  auto store = [&](const Tile& g) {
+    double* __restrict__ dst = out + g.base;
+    if (g.vec) {
+#pragma unroll
+      for (int i = 0; i < PFO; ++i) {
+        const int idx = tid + i * BATCH_NT;
+        const int vr = idx % VPO, el = idx / VPO;
+        if (FULL || el < n) {
+          nt_dbl2 v;
+          v.x = lds[at(vr * 2, el)];
+          v.y = lds[at(vr * 2 + 1, el)];
+          __builtin_nontemporal_store(v, reinterpret_cast<nt_dbl2*>(dst + static_cast<int64_t>(el) * estride + vr * 2));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PFS; ++i) {
+        const int idx = tid + i * BATCH_NT;
+        const int r = idx % BATCH_TS, el = idx / BATCH_TS;
+        if (el < n && r < g.rows)
+          __builtin_nontemporal_store(lds[at(r, el)], dst + r + static_cast<int64_t>(el) * estride);
+      }
+    }
+  };
+
+  int64_t t = t0;
+  Tile g = geometry(t);
+  fetch(g, 0, PFV);
+  for (;;) {
+    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (BATCH_NT - 1);   // the mask keeps its range known
+    __syncthreads();                             // the LDS is free (previous tile stored, previous chunk done)
+    deposit(g);
+    const bool more = t + 1 < t1;                // uniform
+    Tile gn = g;
+    if (more) {
+      gn = geometry(t + 1);
+      fetch(gn, 0, PF_EARLY);                    // the next tile's loads fly during this one
+    }
+    __syncthreads();
+    solve_tile(g);
+    __syncthreads();
+    if (PF_EARLY < PFV && more) fetch(gn, PF_EARLY, PFV);
+    // the store offsets from a fresh opaque index: not computed before the
+    // solve and held through it
+    tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (BATCH_NT - 1);
+    store(g);
+    if (!more) break;
+    ++t;
+    g = gn;
+  }
+}
+
+template <int CLS>
+__device__ __forceinline__ void run_class(CJob& J, const NtPlanK& P, int64_t t0, int64_t t1, double* lds) {
+  constexpr int KIND = CLS % 3;
+  constexpr int PREC = (CLS / 3) % 2;
+  constexpr bool FULL = ((CLS / 6) % 2) != 0;
+  constexpr int R = 1 << ((CLS / 12) % 4);
+  constexpr int LAYOUT = CLS / 48;
+  if constexpr (LAYOUT == CONTIG) contig_chunk<R, FULL, PREC, KIND>(J, P, t0, t1, lds);
+  else strided_chunk<R, FULL, PREC, KIND>(J, P, t0, t1, lds);
+}
+
+// class `cls` (uniform) of [LO, HI): a tree of scalar branches
+template <int LO, int HI>
+__device__ __forceinline__ void dispatch_class(int cls, CJob& J, const NtPlanK& P, int64_t t0, int64_t t1,
+                                               double* lds) {
+  if constexpr (HI - LO == 1) {
+    run_class<LO>(J, P, t0, t1, lds);
+  } else {
+    constexpr int MID = (LO + HI) / 2;
+    if (cls < MID) dispatch_class<LO, MID>(cls, J, P, t0, t1, lds);
+    else dispatch_class<MID, HI>(cls, J, P, t0, t1, lds);
+  }
+}
+
+// Persistent: each workgroup takes chunks from the work counter until none is
+// left.  The next chunk's index is requested when a chunk starts and read at
+// its end (one workgroup barrier per chunk), so the counter's round trip is
+// hidden.  Chunks are handed out in increasing order, so the job of a chunk is
+// found by a binary search that starts at the previous chunk's job.
+#ifndef NT_BATCH_WAVES_PER_EU
+#define NT_BATCH_WAVES_PER_EU 4
+#endif
+__global__ void __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(NT_BATCH_WAVES_PER_EU)))
+batch_kernel(const BatchArgs args) {
+  __shared__ __attribute__((aligned(16))) double lds[BATCH_LDS];
+  __shared__ unsigned int s_chunk[2];          // chunk indices, alternating slots
+  CJob* jobs = (CJob*)(args.jobs);
+  CPlan* plans = (CPlan*)(args.plans);
+  if (threadIdx.x == 0) s_chunk[0] = atomicAdd(args.counter, 1u);
+  __syncthreads();
+  int slot = 0;
+  int j = 0;
+  for (;;) {
+    const int64_t c = static_cast<int64_t>(static_cast<unsigned>(uni(static_cast<int>(s_chunk[slot]))));
+    if (c >= args.total) break;                // every wavefront reaches this
+    unsigned int nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(args.counter, 1u);
+    {                                            // the last job with chunk0 <= c, at or after j
+      int lo = j, hi = args.njobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].chunk0 <= c) lo = mid;
+        else hi = mid - 1;
+      }
+      j = lo;
+    }
+    CJob& J = jobs[j];
+    const int64_t t0 = (c - J.chunk0) * J.chunk_tiles;
+    const int64_t t1e = t0 + J.chunk_tiles;
+    const int64_t t1 = t1e < J.ntiles ? t1e : J.ntiles;
+    const NtPlanK& P = *(const NtPlanK*)(&plans[J.plan]);
+#ifdef NT_BATCH_CLASS_LO
+    dispatch_class<NT_BATCH_CLASS_LO, NT_BATCH_CLASS_HI>(J.cls, J, P, t0, t1, lds);
+#else
+    dispatch_class<0, NUM_CLASSES>(J.cls, J, P, t0, t1, lds);
+#endif
+    if (threadIdx.x == 0) s_chunk[slot ^ 1] = nxt;
+    __syncthreads();
+    slot ^= 1;
   }
 }
 
@@ -293,21 +619,27 @@ int launch_batch(const void* dev_image, int njobs, int nplans, int64_t total_til
                  int grid, hipStream_t stream, const char* who) {
   int g = grid;
   if (g <= 0) {
-    int dev = 0, cus = 256;
+    // persistent: as many workgroups as are resident at once
+    int dev = 0, cus = 256, per_cu = 2;
     if (hipGetDevice(&dev) == hipSuccess) {
       hipDeviceProp_t prop;
       if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
-    g = 4 * cus;   // several workgroups per CU keep loads and solves overlapping
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nt::batch_kernel, nt::BATCH_NT, 0) == hipSuccess && nb > 0)
+      per_cu = nb;
+    g = per_cu * cus;
   }
   if (g > total_tiles) g = static_cast<int>(total_tiles);
+  // every chunk and every workgroup's last request take one counter increment
+  if (total_tiles + g >= 0xffffffffLL) return nt::set_error(NT_EINVAL, "%s: too many chunks", who);
   nt::BatchArgs a;
   a.jobs = static_cast<const nt::BatchJob*>(dev_image);
   const size_t jb = sizeof(nt::BatchJob) * static_cast<size_t>(njobs);
   a.plans = reinterpret_cast<const NtPlanK*>(static_cast<const char*>(dev_image) +
                                              ((jb + 255) & ~static_cast<size_t>(255)));
   a.counter = dev_counter;
-  a.total_tiles = total_tiles;
+  a.total = total_tiles;
   a.njobs = njobs;
   a.pad_ = 0;
   hipLaunchKernelGGL(nt::batch_kernel, dim3(g), dim3(nt::BATCH_NT), 0, stream, a);
@@ -366,7 +698,16 @@ int nt_batch_pack(const nt_batch_job* jobs, int njobs, const nt_plan* const* pla
     if (!plans[i]) return nt::set_error(NT_EINVAL, "nt_batch_pack: plan %d is null", i);
     Pk[i] = plans[i]->k;
   }
-  int64_t tile0 = 0;
+  double total_pts = 0.0;                       // (a double: extents are validated below)
+  for (int i = 0; i < njobs; ++i)
+    if (jobs[i].nz > 0 && jobs[i].ny > 0 && jobs[i].nx > 0)
+      total_pts += static_cast<double>(jobs[i].nz) * static_cast<double>(jobs[i].ny) * static_cast<double>(jobs[i].nx);
+  const double want = total_pts / static_cast<double>(nt::BATCH_CHUNKS_PER_SLOT * 512);
+  int64_t chunk_pts = (want >= static_cast<double>(nt::BATCH_CHUNK_MAX_PTS)) ? nt::BATCH_CHUNK_MAX_PTS
+                                                                                : static_cast<int64_t>(want);
+  if (chunk_pts > nt::BATCH_CHUNK_MAX_PTS) chunk_pts = nt::BATCH_CHUNK_MAX_PTS;
+  if (chunk_pts < nt::BATCH_CHUNK_MIN_PTS) chunk_pts = nt::BATCH_CHUNK_MIN_PTS;
+  int64_t chunk0 = 0;
   for (int i = 0; i < njobs; ++i) {
     const nt_batch_job& q = jobs[i];
     if (!q.in || !q.out) return nt::set_error(NT_EINVAL, "nt_batch_pack: job %d: null buffer", i);
@@ -397,26 +738,41 @@ int nt_batch_pack(const nt_batch_job* jobs, int njobs, const nt_plan* const* pla
     b.in = q.in;
     b.out = q.out;
     int64_t outer;
+    int layout;
     if (q.axis == 2) {
-      b.layout = nt::CONTIG; outer = 1; b.phys = q.nz * q.ny; b.pstride = q.nx; b.estride = 1; b.ostride = 0;
+      layout = nt::CONTIG; outer = 1; b.phys = q.nz * q.ny; b.pstride = q.nx; b.estride = 1; b.ostride = 0;
     } else if (q.axis == 1) {
-      b.layout = nt::STRIDED; outer = q.nz; b.phys = q.nx; b.pstride = 1; b.estride = q.nx; b.ostride = q.ny * q.nx;
+      layout = nt::STRIDED; outer = q.nz; b.phys = q.nx; b.pstride = 1; b.estride = q.nx; b.ostride = q.ny * q.nx;
     } else {
-      b.layout = nt::STRIDED; outer = 1; b.phys = q.ny * q.nx; b.pstride = 1; b.estride = q.ny * q.nx; b.ostride = 0;
+      layout = nt::STRIDED; outer = 1; b.phys = q.ny * q.nx; b.pstride = 1; b.estride = q.ny * q.nx; b.ostride = 0;
     }
     b.tiles = (b.phys + nt::BATCH_TS - 1) / nt::BATCH_TS;
     b.ntiles = outer * b.tiles;
-    b.tile0 = tile0;
-    tile0 += b.ntiles;
+    int64_t ct = chunk_pts / (nt::BATCH_TS * n);
+    if (ct < 1) ct = 1;
+    b.chunk_tiles = ct;
+    b.chunk0 = chunk0;
+    chunk0 += (b.ntiles + ct - 1) / ct;
     b.s1 = (q.producer == 1) ? 3.0 / (4.0 * q.dx) : 0.0;
     b.s2 = (q.producer == 1) ? 1.0 / (2.0 * q.dx) : 0.0;
     b.plan = q.plan_slot;
-    b.prod = (q.producer == 1) ? nt::P_COMPACT : nt::P_IDENT;
-    b.in_f32 = (q.in_dtype == NT_F32);
-    b.prec = (q.precision == NT_PREC_F64) ? nt::PREC_F64 : nt::PREC_F32R;
+    const int kind = (q.producer == 0) ? nt::K_SOLVE : ((q.in_dtype == NT_F32) ? nt::K_CD32 : nt::K_CD64);
+    const int rsel = (n <= 64) ? 0 : (n <= 128) ? 1 : (n <= 256) ? 2 : 3;
+    const int full = (n == (int64_t(64) << rsel)) ? 1 : 0;
+    const int prec = (q.precision == NT_PREC_F64) ? nt::PREC_F64 : nt::PREC_F32R;
+    b.cls = nt::batch_class(layout, rsel, full, prec, kind);
     b.sweeps = q.sweeps;
+    // 16-byte accesses: aligned buffers, and every line (contiguous) or tile
+    // row (strided) starting on a 16-byte boundary
+    const int64_t isz = (q.in_dtype == NT_F32) ? 4 : 8;
+    const bool al = reinterpret_cast<uintptr_t>(q.in) % 16 == 0 && reinterpret_cast<uintptr_t>(q.out) % 16 == 0;
+    if (layout == nt::CONTIG)
+      b.vec = al && (b.pstride * isz) % 16 == 0 && (b.pstride * 8) % 16 == 0;
+    else
+      b.vec = al && (b.estride * isz) % 16 == 0 && (b.ostride * isz) % 16 == 0 && (b.estride * 8) % 16 == 0 &&
+              (b.ostride * 8) % 16 == 0;
   }
-  *total_tiles = tile0;
+  *total_tiles = chunk0;
   return nt::set_error(NT_OK, "%s", "");
 }
 
