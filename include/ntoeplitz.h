@@ -149,6 +149,17 @@ int nt_part_plan_secondary(const nt_part_plan* plan, int p, double* xU, double* 
 int nt_part_solve_lines(const nt_part_plan* plan, int producer, const void* in, void* out,
                         int64_t lines, double p1, double p2, void* stream);
 
+/* The partitioned solve along `axis` (0 = z, 1 = y, 2 = x) of a C-contiguous
+ * (nz, ny, nx) FP64 field: the reference's permute + partitioned solve +
+ * permute back for y and z (run-parallel-compact.py:92-164) without the
+ * permutes.  Strided lines are staged in tiles of 8 or 16 adjacent lines.
+ * producer 0: solve in place (in == out); 1: compact derivative, p1 = dx;
+ * 2: an ADI half step of an n x n field (nz = 1) in -> out, p1 = r: axis 2 is
+ * the x-implicit half step (rows), axis 1 the y-implicit one (columns); lines
+ * 0 and n-1 are not solved. */
+int nt_part_solve_axis(const nt_part_plan* plan, int producer, const void* in, void* out, int64_t nz,
+                       int64_t ny, int64_t nx, int axis, double p1, void* stream);
+
 /* The partition method one step at a time, as the reference's ranks run it
  * (run-parallel-compact.py:72-76).  Subdomain arrays are FP64, C-contiguous.
  *

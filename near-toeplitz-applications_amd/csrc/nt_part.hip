@@ -54,6 +54,25 @@ bool thomas(int n, const double* a, const double* b, const double* c, const doub
 
 }  // namespace
 
+namespace {
+
+// Segment of W lanes x R rows per subdomain (W R >= m).  Many subdomains per
+// line (P >= 12) or short ones (m <= 64): 16-lane segments (four subdomains per
+// wavefront, R <= 8) up to 128 points, 32 lanes up to 256, fewer wavefronts per
+// line.  Otherwise one 64-lane wavefront per subdomain: in the one-line kernel
+// the first and last subdomains have a wavefront each, so at small P packed
+// segments would sit idle.  Measured on MI355X: P = 16, m = 128: 0.296 ms
+// (segments) against 0.400 ms on 128^2 lines; P = 4, m = 128: 0.132 ms against
+// 0.104 ms.
+void pick_segments(int P, int m, int* R, int* W) {
+  *R = 1;
+  *W = 64;
+  if ((P >= 12 || m <= 64) && m <= 32 * 8) *W = (m <= 16 * 8) ? 16 : 32;
+  while (*W * *R < m) *R *= 2;
+}
+
+}  // namespace
+
 extern "C" {
 
 int nt_part_plan_create(int64_t n, int P, const double coeffs[7], nt_part_plan** out) {
@@ -239,21 +258,8 @@ int nt_part_solve_lines(const nt_part_plan* pp, int producer, const void* in, vo
   }
   if (grid <= 0) return nt::set_error(NT_OK, "%s", "");
   if (grid > 0x7fffffffLL) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_lines: too many lines");
-  // Segment of W lanes x R rows per subdomain (W R >= m).  Many subdomains per
-  // line (P >= 12) or short ones (m <= 64): 16-lane segments (four subdomains per
-  // wavefront, R <= 8) up to 128 points, 32 lanes up to 256, fewer wavefronts per
-  // line.  Otherwise one 64-lane wavefront per subdomain: the first and last
-  // subdomains have a wavefront each, so at small P packed segments would sit
-  // idle.  Measured on MI355X: P = 16, m = 128: 0.296 ms (segments) against
-  // 0.400 ms on 128^2 lines; P = 4, m = 128: 0.132 ms against 0.104 ms.
-  const int m = pp->m;
   int R = 1, W = 64;
-  if ((pp->P >= 12 || m <= 64) && m <= 32 * 8) {
-    W = (m <= 16 * 8) ? 16 : 32;
-    while (W * R < m) R *= 2;
-  } else {
-    while (W * R < m) R *= 2;
-  }
+  pick_segments(pp->P, pp->m, &R, &W);
   const dim3 g(static_cast<unsigned>(grid)), b(64 * nt::part_waves(pp->P, W));
   hipStream_t st = static_cast<hipStream_t>(stream);
 #define NT_PGO(RR, WW)                                                                   \
@@ -287,6 +293,111 @@ int nt_part_solve_lines(const nt_part_plan* pp, int producer, const void* in, vo
     std::snprintf(buf, sizeof(buf), "nt_part_solve_lines: %s", hipGetErrorString(e));
     return nt::set_error(NT_EHIP, "%s", buf);
   }
+  return nt::set_error(NT_OK, "%s", "");
+}
+
+int nt_part_solve_axis(const nt_part_plan* pp, int producer, const void* in, void* out, int64_t nz, int64_t ny,
+                       int64_t nx, int axis, double p1, void* stream) {
+  if (!pp || !in || !out) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: null argument");
+  if (axis < 0 || axis > 2) return nt::set_error(NT_EINVAL, "nt_part_solve_axis: axis=%d", axis);
+  if (nz <= 0 || ny <= 0 || nx <= 0 || nz > (int64_t(1) << 40) / ny / nx)
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: bad extents");
+  if (producer < 0 || producer > 2)
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: producer must be 0, 1 or 2");
+  const int64_t n = (axis == 2) ? nx : (axis == 1 ? ny : nz);
+  if (n != pp->n)
+    return nt::set_error(NT_EINVAL, "nt_part_solve_axis: line length %lld != plan n %lld", (long long)n,
+                         (long long)pp->n);
+  if (producer == 2 && !(nz == 1 && ny == pp->n && nx == pp->n && axis >= 1))
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: ADI half steps take an n x n field (nz = 1), axis 2 or 1");
+  if (producer != 0 && in == out)
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: in and out must not alias");
+  if (producer == 0 && in != out)
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: solves are in place (in == out)");
+  // contiguous x-lines: the one-line-per-workgroup kernel
+  if (axis == 2) return nt_part_solve_lines(pp, producer, in, out, (producer == 2) ? pp->n : nz * ny, p1, 0.0, stream);
+
+  if (!pp->dev) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: plan has no device copy (no HIP device at creation)");
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != pp->device)
+    return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: plan was created on another device");
+  nt::PartTileArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.dev = static_cast<const nt::NtPartDev*>(pp->dev);
+  const double* tail =
+      reinterpret_cast<const double*>(static_cast<const char*>(pp->dev) + sizeof(nt::NtPartDev));
+  a.xu = tail;
+  a.xl = tail + 3 * pp->m;
+  a.in = in;
+  a.out = out;
+  int64_t outer;
+  if (axis == 1) {
+    outer = nz; a.phys = nx; a.estride = nx; a.ostride = ny * nx;
+  } else {
+    outer = 1; a.phys = ny * nx; a.estride = ny * nx; a.ostride = 0;
+  }
+  if (producer == 1) {
+    if (!(std::isfinite(p1) && p1 != 0.0)) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: dx");
+    a.s1 = 3.0 / (4.0 * p1);
+    a.s2 = 1.0 / (2.0 * p1);
+  } else if (producer == 2) {
+    const double* c = pp->coeffs;
+    if (c[0] != 1.0 || c[1] != 0.0 || c[5] != 0.0 || c[6] != 1.0)
+      return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: ADI plan needs identity end rows");
+    a.s1 = 0.5 * p1;
+    a.s2 = 1.0 - p1;
+  }
+  const int64_t nsys = a.phys - ((producer == 2) ? 2 : 0);
+  if (nsys <= 0) return nt::set_error(NT_OK, "%s", "");
+  int R = 1, W = 64;
+  pick_segments(pp->P, pp->m, &R, &W);
+  const int CP = (R >= 2) ? 1 : 0;
+  const int LR = (R == 1) ? 0 : (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : (R == 16) ? 4 : 5;
+  const int lp = static_cast<int>(pp->n + (pp->n >> LR) * CP + 2);
+  auto lds_bytes = [&](int tl) { return static_cast<size_t>(tl) * (lp + 4 * pp->P) * sizeof(double); };
+  // 16 lines per tile (128-byte rows) where two workgroups still share a CU, else 8,
+  // else 4 (2048-point lines of 256-point subdomains)
+  const int tls = (lds_bytes(16) <= 80 * 1024) ? 4 : ((lds_bytes(8) <= 160 * 1024) ? 3 : 2);
+  const int TL = 1 << tls;
+  if (lds_bytes(TL) > 160 * 1024)
+    return nt::set_error(NT_EUNSUPPORTED, "%s", "nt_part_solve_axis: tile does not fit in LDS");
+  a.tl_log2 = tls;
+  a.lp = lp;
+  a.tiles = (nsys + TL - 1) / TL;
+  const int64_t grid = outer * a.tiles;
+  if (grid > 0x7fffffffLL) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: too many lines");
+  const int S = 64 / W;
+  const int ntasks = pp->P * TL;
+  int waves = (ntasks + S - 1) / S;
+  if (waves > 16) waves = 16;
+  const dim3 g(static_cast<unsigned>(grid)), b(64 * waves);
+  const size_t shm = lds_bytes(TL);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define NT_PTGO(RR, WW)                                                                          \
+  if (R == (RR) && W == (WW)) {                                                                  \
+    if (producer == 0)                                                                           \
+      hipLaunchKernelGGL((nt::part_tile_kernel<RR, WW, nt::P_IDENT>), g, b, shm, st, a);         \
+    else if (producer == 1)                                                                      \
+      hipLaunchKernelGGL((nt::part_tile_kernel<RR, WW, nt::P_COMPACT>), g, b, shm, st, a);       \
+    else                                                                                         \
+      hipLaunchKernelGGL((nt::part_tile_kernel<RR, WW, nt::P_ADI>), g, b, shm, st, a);           \
+  } else
+  NT_PTGO(1, 16)
+  NT_PTGO(2, 16)
+  NT_PTGO(4, 16)
+  NT_PTGO(8, 16)
+  NT_PTGO(8, 32)
+  NT_PTGO(1, 64)
+  NT_PTGO(2, 64)
+  NT_PTGO(4, 64)
+  NT_PTGO(8, 64)
+  NT_PTGO(16, 64)
+  NT_PTGO(32, 64) {
+    return nt::set_error(NT_EUNSUPPORTED, "%s", "nt_part_solve_axis: subdomain too large");
+  }
+#undef NT_PTGO
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return nt::set_error(NT_EHIP, "nt_part_solve_axis: %s", hipGetErrorString(e));
   return nt::set_error(NT_OK, "%s", "");
 }
 

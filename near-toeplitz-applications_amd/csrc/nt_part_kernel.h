@@ -221,4 +221,182 @@ __global__ void __launch_bounds__(part_max_threads<R>()) part_kernel(const PartA
   }
 }
 
+// ---- strided lines (y- and z-lines, the ADI y-implicit half step) ------------------------
+// A workgroup owns a tile of TL adjacent strided lines (TL = 8 or 16, a power of
+// two): their rows are runs of TL consecutive elements in memory, read with
+// coalesced accesses.  The line's right-hand side is formed while staging (for
+// the compact derivative from the point and its neighbours along the line, for
+// ADI from the neighbouring lines at the same position), so the subdomain
+// solves read only their own chunks and overwrite them with their solutions.
+// The TL x P subdomain solves are spread over the wavefronts, S = 64/W per
+// wavefront, ordered subdomain-major so that the S segments of a wavefront
+// share one subdomain index and with it one (uniform) level table.  The
+// reduced systems of all TL lines are solved together, and the correction
+// x = xR + alpha xU + beta xL is applied while the tile is stored.
+//
+// LDS (dynamic): line l, point e -> l*LP + e + (e >> log2 R) * CP, with
+// LP = n + (n >> log2 R) * CP + 2; then the faces and the reduced solutions,
+// TL x 2P doubles each.
+struct PartTileArgs {
+  const NtPartDev* dev;
+  const double* xu;     // xU[kind*m + i]
+  const double* xl;     // xL[kind*m + i]
+  const void* in;
+  void* out;
+  int64_t estride;      // between the points of a line (a row of the tile)
+  int64_t ostride;      // between slabs
+  int64_t phys;         // lines per slab, including ADI boundary lines
+  int64_t tiles;        // tiles per slab
+  double s1, s2;        // producer scalars as in LineArgs
+  int32_t tl_log2;      // log2 of the lines per tile
+  int32_t lp;           // LDS doubles per staged line
+};
+
+template <int R>
+__host__ __device__ constexpr int part_tile_cp() {
+  return (R >= 2) ? 1 : 0;
+}
+
+template <int R, int W, int PROD>
+__global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args) {
+  extern __shared__ double lds_dyn[];
+  constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
+  constexpr int S = 64 / W;                      // subdomains per wavefront
+  constexpr int CP = part_tile_cp<R>();
+  constexpr int LR = Log2<R>::value;
+  constexpr int LW = Log2<W>::value;
+  const NtPartDev& D = *args.dev;
+  const int m = D.m, P = D.P, n = D.n;
+  const int TLS = args.tl_log2;
+  const int TL = 1 << TLS;
+  const int LP = args.lp;
+  const int NP2 = 2 * P;
+  double* __restrict__ lds = lds_dyn;
+  double* __restrict__ red = lds_dyn + TL * LP;  // faces, TL x 2P
+  double* __restrict__ sol = red + TL * NP2;     // reduced solutions, TL x 2P
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int nw = nt >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int seg = lane >> LW;
+  const int sl = lane & (W - 1);
+  auto at = [&](int l, int e) -> int { return l * LP + e + (e >> LR) * CP; };
+
+  // tile geometry: solved lines p0 .. p0 + rows - 1 of slab o (staged index + HALO)
+  const int64_t nsys = args.phys - 2 * HALO;
+  const int64_t t = blockIdx.x;
+  const int64_t o = t / args.tiles;
+  const int64_t p0 = (t - o * args.tiles) << TLS;
+  const int64_t rv = nsys - p0;
+  const int rows = rv < TL ? static_cast<int>(rv) : TL;
+  const int64_t base = o * args.ostride + p0 + HALO;
+  const double* __restrict__ in = static_cast<const double*>(args.in) + base;
+  double* __restrict__ out = static_cast<double*>(args.out) + base;
+  const int64_t es = args.estride;
+
+  // ---- stage the right-hand side, TL points of a row per TL consecutive threads --
+  const int total = n << TLS;
+  for (int i0 = tid; i0 < total; i0 += 4 * nt) {
+    double c[4], a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = i0 + u * nt;
+      const int l = idx & (TL - 1), e = idx >> TLS;
+      const bool ok = idx < total && l < rows;
+      c[u] = a[u] = b[u] = 0.0;
+      if (ok) {
+        const double* p = in + static_cast<int64_t>(e) * es + l;
+        c[u] = __builtin_nontemporal_load(p);
+        if (PROD == P_ADI) {                     // neighbouring lines: the row's next elements
+          a[u] = p[-1];
+          b[u] = p[1];
+        } else if (PROD == P_COMPACT) {          // neighbours along the line (closures at its ends)
+          const int e1 = (e == 0) ? 1 : ((e == n - 1) ? n - 2 : e - 1);
+          const int e2 = (e == 0) ? 2 : ((e == n - 1) ? n - 3 : e + 1);
+          a[u] = in[static_cast<int64_t>(e1) * es + l];
+          b[u] = in[static_cast<int64_t>(e2) * es + l];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = i0 + u * nt;
+      const int l = idx & (TL - 1), e = idx >> TLS;
+      if (idx < total && l < rows) {
+        double v = c[u];
+        if (PROD == P_ADI) {
+          v = (e == 0 || e == n - 1) ? c[u] : args.s1 * (a[u] + b[u]) + args.s2 * c[u];
+        } else if (PROD == P_COMPACT) {
+          // interior 3/(4h) (f[e+1] - f[e-1]); closures (parallelcompact/rhs.py:42-52)
+          if (e == 0) v = args.s2 * (-5.0 * c[u] + 4.0 * a[u] + b[u]);
+          else if (e == n - 1) v = -args.s2 * (-5.0 * c[u] + 4.0 * a[u] + b[u]);
+          else v = args.s1 * (b[u] - a[u]);
+        }
+        lds[at(l, e)] = v;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- subdomain solves: task tau = q * TL + l, S per wavefront ----------------------
+  const int ntasks = P << TLS;
+  const int last_lane = (m - 1) / R, last_k = (m - 1) % R;
+  for (int wt = w; wt * S < ntasks; wt += nw) {
+    const int tau0 = wt * S;                     // uniform
+    const int q = tau0 >> TLS;                   // the same for the S segments
+    const int l = (tau0 & (TL - 1)) + seg;
+    const int kind = (P == 1) ? 0 : (q == 0 ? 0 : (q == P - 1 ? 2 : 1));
+    const bool active = l < rows;
+    const int e0 = q * m + sl * R;
+    double d[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int el = sl * R + k;
+      d[k] = (active && el < m) ? lds[at(l, e0 + k)] : 0.0;
+    }
+    cr_solve_regs<double, R, W>(d, D.plan[kind], D.nlev, sl);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int el = sl * R + k;
+      if (active && el < m) lds[at(l, e0 + k)] = d[k];
+    }
+    // faces: negated first/last value of the subdomain (reduced.py:51-88)
+    if (active && sl == 0) red[l * NP2 + 2 * q] = (q == 0) ? 0.0 : -d[0];
+    double v = d[0];
+#pragma unroll
+    for (int k = 1; k < R; ++k) v = (k == last_k) ? d[k] : v;
+    if (active && sl == last_lane) red[l * NP2 + 2 * q + 1] = (q == P - 1) ? 0.0 : -v;
+  }
+  __syncthreads();
+
+  // ---- reduced systems: (alpha_0, beta_0, alpha_1, ...) = inverse * faces, per line --
+  for (int i = tid; i < (NP2 << TLS); i += nt) {
+    const int l = i / NP2, r = i - l * NP2;
+    double u = 0.0;
+    if (l < rows) {
+      const double* __restrict__ row = D.red_inv + r * NP2;
+      const double* __restrict__ f = red + l * NP2;
+      for (int j = 0; j < NP2; ++j) u += row[j] * f[j];
+    }
+    sol[i] = u;
+  }
+  __syncthreads();
+
+  // ---- correction x = xR + alpha xU + beta xL (sum.py:30) while storing ---------------
+  // the subdomain of point e: (e + 1/2) / m in FP32 is exact for e < 2^12 points
+  const float inv_m = 1.0f / static_cast<float>(m);
+  for (int idx = tid; idx < total; idx += nt) {
+    const int l = idx & (TL - 1), e = idx >> TLS;
+    if (l < rows) {
+      const int q = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_m);
+      const int el = e - q * m;
+      const int kind = (P == 1) ? 0 : (q == 0 ? 0 : (q == P - 1 ? 2 : 1));
+      const double x = lds[at(l, e)] + sol[l * NP2 + 2 * q] * args.xu[kind * m + el] +
+                       sol[l * NP2 + 2 * q + 1] * args.xl[kind * m + el];
+      __builtin_nontemporal_store(x, out + static_cast<int64_t>(e) * es + l);
+    }
+  }
+}
+
 }  // namespace nt

@@ -56,6 +56,7 @@ SIGNATURES = {
     "nt_part_plan_destroy": (_int, [_vp]),
     "nt_part_plan_secondary": (_int, [_vp, _int, _dp, _dp, _int]),
     "nt_part_solve_lines": (_int, [_vp, _int, _vp, _vp, _i64, _dbl, _dbl, _vp]),
+    "nt_part_solve_axis": (_int, [_vp, _int, _vp, _vp, _i64, _i64, _i64, _int, _dbl, _vp]),
     "nt_part_rhs_ghosted": (_int, [_vp, _vp, _i64, _i64, _i64, _dbl, _int, _int, _vp]),
     "nt_part_params": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _vp]),
     "nt_part_sum": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),

@@ -7,6 +7,10 @@ Reference pieces and their counterparts here:
   parallelcompact/reduced.py:135-196 get_params             -> reduced 2P system, in-kernel
   parallelcompact/sum.py:7-51      sum_solutions            -> correction, in-kernel
 One launch handles a batch of lines; no host round trip, no per-call allocation.
+Contiguous x-lines take one line per workgroup; y- and z-lines (and the ADI
+y-implicit half step) are staged in tiles of adjacent lines, so the
+reference's permute before and after the partitioned solve
+(experiments/run-parallel-compact.py:92-164) never happens.
 """
 import ctypes
 
@@ -15,7 +19,7 @@ import numpy as np
 from . import _lib
 from .core import device_ptr, stream_handle, storage_elems_from
 
-__all__ = ["PartitionPlan"]
+__all__ = ["PartitionPlan", "PartitionedADISolver"]
 
 PRODUCER_SOLVE = 0
 PRODUCER_COMPACT = 1
@@ -67,6 +71,33 @@ class PartitionPlan:
                   f.numel() // self.n, dx)
         return out
 
+    def _axis(self, producer, src, dst, shape, axis, p1=0.0):
+        nz, ny, nx = (int(v) for v in shape)
+        _lib.call("nt_part_solve_axis", self._h, int(producer), src, dst, ctypes.c_int64(nz),
+                  ctypes.c_int64(ny), ctypes.c_int64(nx), int(axis), float(p1), stream_handle())
+
+    def solve_axis(self, x, axis):
+        """Solve every line along ``axis`` (0 z, 1 y, 2 x) of a 3-D field in place."""
+        if x.dim() != 3 or x.shape[axis] != self.n:
+            raise _lib.NtInvalidArgument(_lib.NT_EINVAL, "x must be 3-D with extent n along axis")
+        p = device_ptr(x, "x")
+        self._axis(PRODUCER_SOLVE, p, p, x.shape, axis)
+        return x
+
+    def compact_derivative(self, f, out, dx, axis=2):
+        """Compact derivative along ``axis`` (0 z, 1 y, 2 x) of a 3-D field; f, out same shape."""
+        if f.dim() != 3 or tuple(f.shape) != tuple(out.shape) or f.shape[axis] != self.n:
+            raise _lib.NtInvalidArgument(_lib.NT_EINVAL, "f/out must be 3-D, same shape, extent n along axis")
+        self._axis(PRODUCER_COMPACT, device_ptr(f, "f"), device_ptr(out, "out"), f.shape, axis, dx)
+        return out
+
+    def adi_half_y(self, work, u, r):
+        """y-implicit PR half step work -> u (columns 1..n-2) of an n x n field."""
+        if tuple(u.shape) != (self.n, self.n) or tuple(work.shape) != (self.n, self.n):
+            raise _lib.NtInvalidArgument(_lib.NT_EINVAL, "u and work must be n x n")
+        self._axis(PRODUCER_ADI, device_ptr(work, "work"), device_ptr(u, "u"), (1, self.n, self.n), 1, r)
+        return u
+
     def adi_half_x(self, u, work, r):
         """x-implicit PR half step u -> work (rows 1..n-2) of an n x n field."""
         if tuple(u.shape) != (self.n, self.n) or tuple(work.shape) != (self.n, self.n):
@@ -82,3 +113,63 @@ class PartitionPlan:
             except Exception:
                 pass
             self._h = None
+
+
+class PartitionedADISolver:
+    """Peaceman-Rachford ADI steps with both half steps partitioned: every line
+    of the N x N grid is cut into P subdomains, solved as a batch, and joined
+    through the reduced 2P-unknown system (the one-GPU share of the reference's
+    distributed solve).  A step is two launches, the x-implicit half step
+    u -> work (rows) and the y-implicit one work -> u (columns); ``capture``
+    records k steps into a hipGraph (``torch.cuda.CUDAGraph``)."""
+
+    def __init__(self, N, r, P=16):
+        from . import adi_coeffs
+        self.N, self.r, self.P = int(N), float(r), int(P)
+        self.plan = PartitionPlan(self.N, self.P, adi_coeffs(self.r))
+        self.work = None
+        self._graph = None
+
+    def _work_for(self, u):
+        import torch
+        if self.work is None or self.work.device != u.device:
+            self.work = torch.empty((self.N, self.N), dtype=torch.float64, device=u.device)
+        return self.work
+
+    def prepare(self, u):
+        """Copy u (with its Dirichlet boundary) into the work buffer."""
+        from .core import adi_prepare
+        adi_prepare(u, self._work_for(u))
+
+    def _steps(self, u, nsteps):
+        for _ in range(int(nsteps)):
+            self.plan.adi_half_x(u, self.work, self.r)
+            self.plan.adi_half_y(self.work, u, self.r)
+
+    def step(self, u, nsteps=1, prepare=True):
+        """Advance u (N x N float64, boundary = Dirichlet data) by nsteps, in place."""
+        if prepare:
+            self.prepare(u)
+        self._steps(u, nsteps)
+        return u
+
+    def capture(self, u, nsteps):
+        """Record nsteps steps on u into a graph (u and the work buffer stay bound)."""
+        import torch
+        self.prepare(u)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self._steps(u, nsteps)
+        torch.cuda.current_stream().wait_stream(s)
+        self._graph, self._graph_u = g, u
+        return g
+
+    def replay(self):
+        if self._graph is None:
+            raise RuntimeError("capture() first")
+        self._graph.replay()
+        return self._graph_u
