@@ -313,9 +313,11 @@ def adi_case(N, r, nsteps, reps):
     w = torch.empty_like(u)
     core_prepare(u, w)
     tx = time_launches(lambda: pp.adi_half_x(u, w, r), 20, 3)
+    tx1 = time_launches(lambda: pp.adi_half_x(u, w, r, tiles=False), 20, 3)
     u2 = torch.empty_like(u)
     ty = time_launches(lambda: pp.adi_half_y(w, u2, r), 20, 3)
     res["partitioned_x_half_step_16x128_us"] = tx["mean_s"] * 1e6
+    res["partitioned_x_half_step_one_row_per_workgroup_us"] = tx1["mean_s"] * 1e6
     res["partitioned_y_half_step_16x128_us"] = ty["mean_s"] * 1e6
     res.update(partitioned_adi_case(u0, r, nsteps, reps))
     return res

@@ -314,9 +314,6 @@ int nt_part_solve_axis(const nt_part_plan* pp, int producer, const void* in, voi
     return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: in and out must not alias");
   if (producer == 0 && in != out)
     return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: solves are in place (in == out)");
-  // contiguous x-lines: the one-line-per-workgroup kernel
-  if (axis == 2) return nt_part_solve_lines(pp, producer, in, out, (producer == 2) ? pp->n : nz * ny, p1, 0.0, stream);
-
   if (!pp->dev) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: plan has no device copy (no HIP device at creation)");
   int cur = -1;
   if (hipGetDevice(&cur) != hipSuccess || cur != pp->device)
@@ -331,10 +328,13 @@ int nt_part_solve_axis(const nt_part_plan* pp, int producer, const void* in, voi
   a.in = in;
   a.out = out;
   int64_t outer;
-  if (axis == 1) {
-    outer = nz; a.phys = nx; a.estride = nx; a.ostride = ny * nx;
+  const int layout = (axis == 2) ? nt::CONTIG : nt::STRIDED;
+  if (axis == 2) {
+    outer = 1; a.phys = nz * ny; a.lstride = nx; a.estride = 1; a.ostride = 0;
+  } else if (axis == 1) {
+    outer = nz; a.phys = nx; a.lstride = 1; a.estride = nx; a.ostride = ny * nx;
   } else {
-    outer = 1; a.phys = ny * nx; a.estride = ny * nx; a.ostride = 0;
+    outer = 1; a.phys = ny * nx; a.lstride = 1; a.estride = ny * nx; a.ostride = 0;
   }
   if (producer == 1) {
     if (!(std::isfinite(p1) && p1 != 0.0)) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: dx");
@@ -351,6 +351,7 @@ int nt_part_solve_axis(const nt_part_plan* pp, int producer, const void* in, voi
   if (nsys <= 0) return nt::set_error(NT_OK, "%s", "");
   int R = 1, W = 64;
   pick_segments(pp->P, pp->m, &R, &W);
+  const bool full = (pp->m == W * R);
   const int CP = (R >= 2) ? 1 : 0;
   const int LR = (R == 1) ? 0 : (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : (R == 16) ? 4 : 5;
   const int lp = static_cast<int>(pp->n + (pp->n >> LR) * CP + 2);
@@ -373,14 +374,20 @@ int nt_part_solve_axis(const nt_part_plan* pp, int producer, const void* in, voi
   const dim3 g(static_cast<unsigned>(grid)), b(64 * waves);
   const size_t shm = lds_bytes(TL);
   hipStream_t st = static_cast<hipStream_t>(stream);
+#define NT_PTK(RR, WW, PR, LY, FU) hipLaunchKernelGGL((nt::part_tile_kernel<RR, WW, PR, LY, FU>), g, b, shm, st, a)
+#define NT_PTL(RR, WW, PR)                                                                        \
+  do {                                                                                            \
+    if (layout == nt::CONTIG) {                                                                   \
+      if (full) NT_PTK(RR, WW, PR, nt::CONTIG, true); else NT_PTK(RR, WW, PR, nt::CONTIG, false); \
+    } else {                                                                                      \
+      if (full) NT_PTK(RR, WW, PR, nt::STRIDED, true); else NT_PTK(RR, WW, PR, nt::STRIDED, false); \
+    }                                                                                             \
+  } while (0)
 #define NT_PTGO(RR, WW)                                                                          \
   if (R == (RR) && W == (WW)) {                                                                  \
-    if (producer == 0)                                                                           \
-      hipLaunchKernelGGL((nt::part_tile_kernel<RR, WW, nt::P_IDENT>), g, b, shm, st, a);         \
-    else if (producer == 1)                                                                      \
-      hipLaunchKernelGGL((nt::part_tile_kernel<RR, WW, nt::P_COMPACT>), g, b, shm, st, a);       \
-    else                                                                                         \
-      hipLaunchKernelGGL((nt::part_tile_kernel<RR, WW, nt::P_ADI>), g, b, shm, st, a);           \
+    if (producer == 0) NT_PTL(RR, WW, nt::P_IDENT);                                              \
+    else if (producer == 1) NT_PTL(RR, WW, nt::P_COMPACT);                                       \
+    else NT_PTL(RR, WW, nt::P_ADI);                                                              \
   } else
   NT_PTGO(1, 16)
   NT_PTGO(2, 16)
@@ -396,6 +403,8 @@ int nt_part_solve_axis(const nt_part_plan* pp, int producer, const void* in, voi
     return nt::set_error(NT_EUNSUPPORTED, "%s", "nt_part_solve_axis: subdomain too large");
   }
 #undef NT_PTGO
+#undef NT_PTL
+#undef NT_PTK
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return nt::set_error(NT_EHIP, "nt_part_solve_axis: %s", hipGetErrorString(e));
   return nt::set_error(NT_OK, "%s", "");

@@ -221,18 +221,21 @@ __global__ void __launch_bounds__(part_max_threads<R>()) part_kernel(const PartA
   }
 }
 
-// ---- strided lines (y- and z-lines, the ADI y-implicit half step) ------------------------
-// A workgroup owns a tile of TL adjacent strided lines (TL = 8 or 16, a power of
-// two): their rows are runs of TL consecutive elements in memory, read with
-// coalesced accesses.  The line's right-hand side is formed while staging (for
-// the compact derivative from the point and its neighbours along the line, for
-// ADI from the neighbouring lines at the same position), so the subdomain
-// solves read only their own chunks and overwrite them with their solutions.
-// The TL x P subdomain solves are spread over the wavefronts, S = 64/W per
-// wavefront, ordered subdomain-major so that the S segments of a wavefront
-// share one subdomain index and with it one (uniform) level table.  The
-// reduced systems of all TL lines are solved together, and the correction
-// x = xR + alpha xU + beta xL is applied while the tile is stored.
+// ---- tiles of lines: strided (y-, z-lines, the ADI y-implicit half step) and
+// contiguous (x-lines, the ADI x-implicit half step) ------------------------------------
+// A workgroup owns a tile of TL lines (TL = 4, 8 or 16, a power of two) that
+// are adjacent in memory: for strided lines their rows are runs of TL
+// consecutive elements, read with coalesced accesses; contiguous lines are
+// TL consecutive rows, read along each line.  The line's right-hand side is
+// formed while staging (for the compact derivative from the point and its
+// neighbours along the line, for ADI from the neighbouring lines at the same
+// position), so the subdomain solves read only their own chunks and overwrite
+// them with their solutions.  The TL x P subdomain solves are spread over the
+// wavefronts, S = 64/W per wavefront, ordered subdomain-major so that the S
+// segments of a wavefront share one subdomain index and with it one (uniform)
+// level table, and every segment has work.  The reduced systems of all TL
+// lines are solved together; each subdomain's owner then applies the
+// correction x = xR + alpha xU + beta xL to its chunk, and the tile is stored.
 //
 // LDS (dynamic): line l, point e -> l*LP + e + (e >> log2 R) * CP, with
 // LP = n + (n >> log2 R) * CP + 2; then the faces and the reduced solutions,
@@ -243,7 +246,8 @@ struct PartTileArgs {
   const double* xl;     // xL[kind*m + i]
   const void* in;
   void* out;
-  int64_t estride;      // between the points of a line (a row of the tile)
+  int64_t lstride;      // between adjacent lines (strided: 1; contiguous: the line length)
+  int64_t estride;      // between the points of a line (strided: the row stride; contiguous: 1)
   int64_t ostride;      // between slabs
   int64_t phys;         // lines per slab, including ADI boundary lines
   int64_t tiles;        // tiles per slab
@@ -257,7 +261,22 @@ __host__ __device__ constexpr int part_tile_cp() {
   return (R >= 2) ? 1 : 0;
 }
 
-template <int R, int W, int PROD>
+// right-hand side of point e of a staged line from the point c and its two
+// neighbours a, b (compact: along the line, e-1 and e+1, or the closure
+// points; ADI: the neighbouring lines at e)
+template <int PROD>
+__device__ __forceinline__ double part_rhs(const PartTileArgs& args, int e, int n, double a, double c, double b) {
+  if (PROD == P_ADI) return (e == 0 || e == n - 1) ? c : args.s1 * (a + b) + args.s2 * c;
+  if (PROD == P_COMPACT) {
+    // interior 3/(4h) (f[e+1] - f[e-1]); closures (parallelcompact/rhs.py:42-52)
+    if (e == 0) return args.s2 * (-5.0 * c + 4.0 * a + b);
+    if (e == n - 1) return -args.s2 * (-5.0 * c + 4.0 * a + b);
+    return args.s1 * (b - a);
+  }
+  return c;
+}
+
+template <int R, int W, int PROD, int LAYOUT, bool FULL>
 __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args) {
   extern __shared__ double lds_dyn[];
   constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
@@ -266,7 +285,8 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
   constexpr int LR = Log2<R>::value;
   constexpr int LW = Log2<W>::value;
   const NtPartDev& D = *args.dev;
-  const int m = D.m, P = D.P, n = D.n;
+  const int m = FULL ? W * R : D.m;
+  const int P = D.P, n = D.n;
   const int TLS = args.tl_log2;
   const int TL = 1 << TLS;
   const int LP = args.lp;
@@ -290,51 +310,82 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
   const int64_t p0 = (t - o * args.tiles) << TLS;
   const int64_t rv = nsys - p0;
   const int rows = rv < TL ? static_cast<int>(rv) : TL;
-  const int64_t base = o * args.ostride + p0 + HALO;
+  const int64_t ls = args.lstride, es = args.estride;
+  const int64_t base = o * args.ostride + (p0 + HALO) * ls;
   const double* __restrict__ in = static_cast<const double*>(args.in) + base;
   double* __restrict__ out = static_cast<double*>(args.out) + base;
-  const int64_t es = args.estride;
 
-  // ---- stage the right-hand side, TL points of a row per TL consecutive threads --
+  // ---- stage the right-hand side ------------------------------------------------------
+  // neighbours of point (l, e): compact along the line (closure points at its
+  // ends), ADI the neighbouring lines
+  auto nb = [&](int e, int side) -> int64_t {   // element offset of the side-th neighbour
+    if (PROD == P_ADI) return (side == 0) ? -ls : ls;
+    const int en = (e == 0) ? (side == 0 ? 1 : 2) : ((e == n - 1) ? (side == 0 ? n - 2 : n - 3)
+                                                                  : (side == 0 ? e - 1 : e + 1));
+    return static_cast<int64_t>(en - e) * es;
+  };
+  // Loads for PU points per thread are issued before any of them is used.
+  constexpr int PU = 8;
   const int total = n << TLS;
-  for (int i0 = tid; i0 < total; i0 += 4 * nt) {
-    double c[4], a[4], b[4];
+  if (LAYOUT == STRIDED) {
+    // TL points of a row per TL consecutive threads
+    for (int i0 = tid; i0 < total; i0 += PU * nt) {
+      double c[PU], a[PU], b[PU];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int idx = i0 + u * nt;
-      const int l = idx & (TL - 1), e = idx >> TLS;
-      const bool ok = idx < total && l < rows;
-      c[u] = a[u] = b[u] = 0.0;
-      if (ok) {
-        const double* p = in + static_cast<int64_t>(e) * es + l;
-        c[u] = __builtin_nontemporal_load(p);
-        if (PROD == P_ADI) {                     // neighbouring lines: the row's next elements
-          a[u] = p[-1];
-          b[u] = p[1];
-        } else if (PROD == P_COMPACT) {          // neighbours along the line (closures at its ends)
-          const int e1 = (e == 0) ? 1 : ((e == n - 1) ? n - 2 : e - 1);
-          const int e2 = (e == 0) ? 2 : ((e == n - 1) ? n - 3 : e + 1);
-          a[u] = in[static_cast<int64_t>(e1) * es + l];
-          b[u] = in[static_cast<int64_t>(e2) * es + l];
+      for (int u = 0; u < PU; ++u) {
+        const int idx = i0 + u * nt;
+        const int l = idx & (TL - 1), e = idx >> TLS;
+        c[u] = a[u] = b[u] = 0.0;
+        if (idx < total && l < rows) {
+          const double* p = in + static_cast<int64_t>(e) * es + l;
+          c[u] = __builtin_nontemporal_load(p);
+          if (PROD != P_IDENT) {
+            a[u] = p[nb(e, 0)];
+            b[u] = p[nb(e, 1)];
+          }
         }
+      }
+#pragma unroll
+      for (int u = 0; u < PU; ++u) {
+        const int idx = i0 + u * nt;
+        const int l = idx & (TL - 1), e = idx >> TLS;
+        if (idx < total && l < rows) lds[at(l, e)] = part_rhs<PROD>(args, e, n, a[u], c[u], b[u]);
       }
     }
+  } else {
+    // consecutive threads along each line; point idx of the tile is (idx / n, idx % n),
+    // advanced incrementally from one division per thread
+    const int step_l = nt / n, step_e = nt - step_l * n;
+    int l0 = tid / n, e0 = tid - l0 * n;
+    for (int i0 = tid; i0 < total; i0 += PU * nt) {
+      double c[PU], a[PU], b[PU];
+      int lu[PU], eu[PU];
+      int l = l0, e = e0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int idx = i0 + u * nt;
-      const int l = idx & (TL - 1), e = idx >> TLS;
-      if (idx < total && l < rows) {
-        double v = c[u];
-        if (PROD == P_ADI) {
-          v = (e == 0 || e == n - 1) ? c[u] : args.s1 * (a[u] + b[u]) + args.s2 * c[u];
-        } else if (PROD == P_COMPACT) {
-          // interior 3/(4h) (f[e+1] - f[e-1]); closures (parallelcompact/rhs.py:42-52)
-          if (e == 0) v = args.s2 * (-5.0 * c[u] + 4.0 * a[u] + b[u]);
-          else if (e == n - 1) v = -args.s2 * (-5.0 * c[u] + 4.0 * a[u] + b[u]);
-          else v = args.s1 * (b[u] - a[u]);
+      for (int u = 0; u < PU; ++u) {
+        lu[u] = l;
+        eu[u] = e;
+        c[u] = a[u] = b[u] = 0.0;
+        if (l < rows) {
+          const double* p = in + l * ls + e;
+          c[u] = __builtin_nontemporal_load(p);
+          if (PROD != P_IDENT) {
+            a[u] = p[nb(e, 0)];
+            b[u] = p[nb(e, 1)];
+          }
         }
-        lds[at(l, e)] = v;
+        l += step_l;
+        e += step_e;
+        if (e >= n) {
+          e -= n;
+          ++l;
+        }
       }
+      l0 = l;
+      e0 = e;
+#pragma unroll
+      for (int u = 0; u < PU; ++u)
+        if (lu[u] < rows) lds[at(lu[u], eu[u])] = part_rhs<PROD>(args, eu[u], n, a[u], c[u], b[u]);
     }
   }
   __syncthreads();
@@ -351,16 +402,11 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
     const int e0 = q * m + sl * R;
     double d[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int el = sl * R + k;
-      d[k] = (active && el < m) ? lds[at(l, e0 + k)] : 0.0;
-    }
-    cr_solve_regs<double, R, W>(d, D.plan[kind], D.nlev, sl);
+    for (int k = 0; k < R; ++k) d[k] = (active && (FULL || sl * R + k < m)) ? lds[at(l, e0 + k)] : 0.0;
+    cr_solve_regs<double, R, W, FULL>(d, D.plan[kind], D.nlev, sl);
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int el = sl * R + k;
-      if (active && el < m) lds[at(l, e0 + k)] = d[k];
-    }
+    for (int k = 0; k < R; ++k)
+      if (active && (FULL || sl * R + k < m)) lds[at(l, e0 + k)] = d[k];
     // faces: negated first/last value of the subdomain (reduced.py:51-88)
     if (active && sl == 0) red[l * NP2 + 2 * q] = (q == 0) ? 0.0 : -d[0];
     double v = d[0];
@@ -383,18 +429,34 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
   }
   __syncthreads();
 
-  // ---- correction x = xR + alpha xU + beta xL (sum.py:30) while storing ---------------
-  // the subdomain of point e: (e + 1/2) / m in FP32 is exact for e < 2^12 points
-  const float inv_m = 1.0f / static_cast<float>(m);
-  for (int idx = tid; idx < total; idx += nt) {
-    const int l = idx & (TL - 1), e = idx >> TLS;
+  // ---- correction x = xR + alpha xU + beta xL (sum.py:30), by each chunk's owner -------
+  for (int wt = w; wt * S < ntasks; wt += nw) {
+    const int tau0 = wt * S;
+    const int q = tau0 >> TLS;
+    const int l = (tau0 & (TL - 1)) + seg;
+    const int kind = (P == 1) ? 0 : (q == 0 ? 0 : (q == P - 1 ? 2 : 1));
     if (l < rows) {
-      const int q = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_m);
-      const int el = e - q * m;
-      const int kind = (P == 1) ? 0 : (q == 0 ? 0 : (q == P - 1 ? 2 : 1));
-      const double x = lds[at(l, e)] + sol[l * NP2 + 2 * q] * args.xu[kind * m + el] +
-                       sol[l * NP2 + 2 * q + 1] * args.xl[kind * m + el];
-      __builtin_nontemporal_store(x, out + static_cast<int64_t>(e) * es + l);
+      const double alpha = sol[l * NP2 + 2 * q], beta = sol[l * NP2 + 2 * q + 1];
+      const double* __restrict__ xu = args.xu + kind * m + sl * R;
+      const double* __restrict__ xl = args.xl + kind * m + sl * R;
+      const int e0 = q * m + sl * R;
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (FULL || sl * R + k < m) lds[at(l, e0 + k)] += alpha * xu[k] + beta * xl[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- store ---------------------------------------------------------------------------
+  if (LAYOUT == STRIDED) {
+    for (int idx = tid; idx < total; idx += nt) {
+      const int l = idx & (TL - 1), e = idx >> TLS;
+      if (l < rows) __builtin_nontemporal_store(lds[at(l, e)], out + static_cast<int64_t>(e) * es + l);
+    }
+  } else {
+    for (int l = 0; l < rows; ++l) {
+      double* __restrict__ dst = out + l * ls;
+      for (int e = tid; e < n; e += nt) __builtin_nontemporal_store(lds[at(l, e)], dst + e);
     }
   }
 }

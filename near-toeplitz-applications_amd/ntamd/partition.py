@@ -98,11 +98,15 @@ class PartitionPlan:
         self._axis(PRODUCER_ADI, device_ptr(work, "work"), device_ptr(u, "u"), (1, self.n, self.n), 1, r)
         return u
 
-    def adi_half_x(self, u, work, r):
-        """x-implicit PR half step u -> work (rows 1..n-2) of an n x n field."""
+    def adi_half_x(self, u, work, r, tiles=True):
+        """x-implicit PR half step u -> work (rows 1..n-2) of an n x n field:
+        in tiles of rows (``tiles=True``) or one row per workgroup."""
         if tuple(u.shape) != (self.n, self.n) or tuple(work.shape) != (self.n, self.n):
             raise _lib.NtInvalidArgument(_lib.NT_EINVAL, "u and work must be n x n")
-        self._run(PRODUCER_ADI, device_ptr(u, "u"), device_ptr(work, "work"), self.n, r)
+        if tiles:
+            self._axis(PRODUCER_ADI, device_ptr(u, "u"), device_ptr(work, "work"), (1, self.n, self.n), 2, r)
+        else:
+            self._run(PRODUCER_ADI, device_ptr(u, "u"), device_ptr(work, "work"), self.n, r)
         return work
 
     def __del__(self):

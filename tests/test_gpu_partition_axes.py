@@ -33,13 +33,14 @@ SHAPES_NP = [(512, 4), (2048, 16), (96, 3), (64, 16), (1024, 8), (2048, 8), (300
 
 
 @pytest.mark.parametrize("n,P", SHAPES_NP)
-@pytest.mark.parametrize("axis", [1, 0])
-def test_partitioned_compact_strided_equals_oracle(dev, n, P, axis):
-    """Compact derivative along y (several slabs) and z (one slab) with the
-    lines cut into P subdomains; 21 or 37 lines per slab leave ragged tiles."""
+@pytest.mark.parametrize("axis", [2, 1, 0])
+def test_partitioned_compact_tiles_equal_oracle(dev, n, P, axis):
+    """Compact derivative in tiles of lines along x (contiguous rows), y
+    (several slabs) and z (one slab), the lines cut into P subdomains; 21, 63
+    or 37 lines leave ragged tiles."""
     from ntamd.partition import PartitionPlan
     from ntamd.workloads import compact_field
-    shape = (3, n, 21) if axis == 1 else (n, 1, 37)
+    shape = {2: (3, 21, n), 1: (3, n, 21), 0: (n, 1, 37)}[axis]
     f = compact_field(shape, seed=n + P + axis)
     h = 1.0 / (n - 1)
     fd = torch.tensor(f, device=dev)
@@ -186,6 +187,27 @@ def test_partitioned_adi_y_half_step_equals_oracle(dev):
     plan.adi_half_y(w, u, r)
     want = _half_step(w.cpu().numpy().T.copy(), r, cr.banded_solve).T
     assert rel_err(u.cpu().numpy(), want) < 1e-12
+
+
+@pytest.mark.parametrize("N,P", [(2048, 16), (512, 4), (130, 2)])
+def test_partitioned_adi_x_half_step_tiles_equal_one_line_kernel(dev, N, P):
+    """The x-implicit half step in tiles of rows against the one-row-per-
+    workgroup kernel and the oracle's half step."""
+    from ntamd import adi_coeffs, core
+    from ntamd.partition import PartitionPlan
+    from oracle.adi import _half_step
+    r = 0.7
+    u0 = adi.gaussian_ic(N, seed=N + 1)
+    u = torch.tensor(u0, device=dev)
+    w1, w2 = torch.empty_like(u), torch.empty_like(u)
+    core.adi_prepare(u, w1)
+    core.adi_prepare(u, w2)
+    plan = PartitionPlan(N, P, adi_coeffs(r))
+    plan.adi_half_x(u, w1, r, tiles=True)
+    plan.adi_half_x(u, w2, r, tiles=False)
+    want = _half_step(u0, r, cr.banded_solve)
+    assert rel_err(w1.cpu().numpy(), want) < 1e-12
+    assert rel_err(w2.cpu().numpy(), want) < 1e-12
 
 
 def test_partitioned_adi_graph_replay_equals_eager(dev):
