@@ -473,28 +473,40 @@ def mixed_batch_case(count=64, seed=1, reps=5, sizes=None, precision="fp32_refin
 
 
 SERVE_RATES = (500.0, 2000.0, 3000.0)
+SERVE_UPLOAD_RATES = (100.0, 400.0)
 
 
-def serving_case(count=1000, rates=SERVE_RATES, seed=2):
+def serving_case(count=1000, rates=SERVE_RATES, upload_rates=SERVE_UPLOAD_RATES, seed=2):
     """Poisson arrivals (rate req/s, numpy default_rng(2)): compact derivatives
-    of random-phase trig fields (per-axis sizes U[32,512], random axis) and ADI
-    runs (side U[256,2048], 1-20 steps, r U[0.1,2]), FP64 or FP32+refinement
-    each; continuous batching vs one-at-a-time plain FP64, same request stream,
-    each on a warmed-up server."""
+    (per-axis sizes U[32,512], random axis) and ADI runs (side U[256,2048],
+    1-20 steps, r U[0.1,2]), FP64 or FP32+refinement each; continuous batching
+    vs one-at-a-time plain FP64, same request stream, each on a warmed-up
+    server.  Two input sources: device-resident (fields made on the device, at
+    `rates`) and uploads (every input copied from pinned host memory on the
+    request's stream, at `upload_rates`: about 150 MB per request crosses the
+    host link)."""
     from ntamd.serve import Server, make_requests
     out = {"requests": count, "seed": seed}
-    for rate in rates:
-        case = {}
-        for mode in ("continuous", "sequential"):
-            reqs = make_requests(count, rate, seed=seed)
-            srv = Server()
-            srv.warmup()                 # first-launch costs are paid outside the timed stream
-            stats = srv.run(reqs, mode=mode)
-            case[mode] = stats
-            log(f"[bench] serving {rate:.0f} req/s {mode}: p50 {stats['p50_ms']:.2f} ms, "
-                f"p99 {stats['p99_ms']:.2f} ms, {stats['throughput_rps']:.0f} req/s")
-        out[f"rate_{rate:.0f}"] = case
+    for share, rs, tag in ((0.0, rates, "rate"), (1.0, upload_rates, "upload_rate")):
+        for rate in rs:
+            case = {}
+            for mode in ("continuous", "sequential"):
+                reqs = make_requests(count, rate, seed=seed, upload_share=share)
+                srv = Server()
+                srv.warmup()             # first-launch costs are paid outside the timed stream
+                stats = srv.run(reqs, mode=mode)
+                case[mode] = stats
+                log(f"[bench] serving {tag} {rate:.0f} req/s {mode}: p50 {stats['p50_ms']:.2f} ms, "
+                    f"p99 {stats['p99_ms']:.2f} ms, {stats['throughput_rps']:.0f} req/s")
+                del srv
+                torch_empty_cache()
+            out[f"{tag}_{rate:.0f}"] = case
     return out
+
+
+def torch_empty_cache():
+    import torch
+    torch.cuda.empty_cache()
 
 
 def served_summary(serving):
@@ -503,8 +515,9 @@ def served_summary(serving):
     if not serving:
         return None
     out = {"requests": serving["requests"], "arrivals": "Poisson, numpy default_rng(2)"}
+    out["uploads"] = "upload_rate_*: every request's input copied host->device (pinned) on its stream"
     for key, case in serving.items():
-        if key.startswith("rate_"):
+        if key.startswith("rate_") or key.startswith("upload_rate_"):
             out[key] = {m: {k: case[m][k] for k in ("p50_ms", "p99_ms", "throughput_rps")}
                         for m in ("continuous", "sequential")}
     return out

@@ -18,7 +18,7 @@ import torch
 from . import _lib
 from .core import Plan, device_ptr, precision_code, stream_handle
 
-__all__ = ["NtBatchJob", "MixedBatch", "job_table", "run_noalloc"]
+__all__ = ["NtBatchJob", "MixedBatch", "ImagePool", "job_table", "run_noalloc"]
 
 COMPACT = (1.0, 2.0, 0.25, 1.0, 0.25, 2.0, 1.0)
 
@@ -132,3 +132,62 @@ class MixedBatch:
         _lib.call("nt_batch_launch", ctypes.c_void_p(self.image.data_ptr()), self.njobs, self.nplans,
                   ctypes.c_int64(self.total_tiles), ctypes.c_void_p(self.counter.data_ptr()),
                   self.grid, stream_handle())
+
+
+class ImagePool:
+    """Reusable job-image slots for batches built again and again (a serving
+    loop builds one per scheduling pass).  Each slot is a pinned host buffer,
+    a device buffer and a work counter, allocated once.  ``run(jobs)`` packs the
+    jobs straight into a free slot's pinned buffer, uploads it on the current
+    stream, clears the counter with a kernel and launches; the slot is free
+    again once the launch's completion event has passed.  With no free slot,
+    ``run`` waits for the oldest one."""
+
+    def __init__(self, slots=8, slot_bytes=1 << 20, device="cuda"):
+        self.slot_bytes = int(slot_bytes)
+        dev = torch.device(device)
+        self.host = [torch.empty(self.slot_bytes, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
+        self.dev = [torch.empty(self.slot_bytes, dtype=torch.uint8, device=dev) for _ in range(slots)]
+        self.counter = [torch.zeros(1, dtype=torch.int32, device=dev) for _ in range(slots)]
+        for d in self.dev:
+            if d.data_ptr() % 256:
+                raise RuntimeError("device image slot is not 256-byte aligned")
+        self.events = [None] * slots
+        self._order = []          # slots in use, oldest first
+
+    def _slot(self):
+        for i, ev in enumerate(self.events):
+            if ev is None or ev.query():
+                self.events[i] = None
+                if i in self._order:
+                    self._order.remove(i)
+                return i
+        i = self._order.pop(0)    # all in flight: wait for the oldest
+        self.events[i].synchronize()
+        self.events[i] = None
+        return i
+
+    def run(self, jobs, grid=0):
+        """Pack, upload and launch ``jobs`` (as ``MixedBatch``) on the current
+        stream; returns (tensors to keep alive until done, completion event)."""
+        lib = _lib.load()
+        arr, plans, keep, _ = job_table(jobs)
+        pv = (ctypes.c_void_p * len(plans))(*[p.handle.value for p in plans])
+        nbytes = lib.nt_batch_image_bytes(len(jobs), len(plans))
+        if nbytes > self.slot_bytes:
+            raise _lib.NtInvalidArgument(_lib.NT_EINVAL, f"batch image of {nbytes} bytes exceeds the "
+                                                         f"pool's {self.slot_bytes}-byte slots")
+        i = self._slot()
+        total = ctypes.c_int64()
+        _lib.check(lib.nt_batch_pack(arr, len(jobs), pv, len(plans), ctypes.c_void_p(self.host[i].data_ptr()),
+                                     ctypes.c_size_t(nbytes), ctypes.byref(total)))
+        self.dev[i][:nbytes].copy_(self.host[i][:nbytes], non_blocking=True)
+        self.counter[i].zero_()
+        _lib.call("nt_batch_launch", ctypes.c_void_p(self.dev[i].data_ptr()), len(jobs), len(plans),
+                  ctypes.c_int64(total.value), ctypes.c_void_p(self.counter[i].data_ptr()), int(grid),
+                  stream_handle())
+        ev = torch.cuda.Event()
+        ev.record()
+        self.events[i] = ev
+        self._order.append(i)
+        return (keep, plans), ev

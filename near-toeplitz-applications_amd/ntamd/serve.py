@@ -3,21 +3,27 @@
 No reference counterpart: the reference runs one field at a time from its
 drivers (experiments/run-compact.py, run-adi.py).  A request is
 
-* ``deriv``: the compact derivative along ``axis`` of a seeded random-phase
-  trigonometric field of shape (nz, ny, nx), or
+* ``deriv``: the compact derivative along ``axis`` of a field of shape
+  (nz, ny, nx), or
 * ``adi``: ``steps`` Peaceman-Rachford steps on a side x side grid at ratio r,
 
-each in FP64 or FP32 + refinement.  ``Server.run`` replays a request stream
-with its arrival times on the wall clock:
+each in FP64 or FP32 + refinement.  A request's input either crosses from the
+host (``upload``: a slice of a pinned host pool, copied on the request's stream
+before its solve, so the copy is part of its latency and its device copy part of
+its admission) or is made on the device (a seeded random-phase trigonometric
+field, or a Gaussian for ADI: data a previous GPU step left resident).
+``Server.run`` replays a request stream with its arrival times on the wall
+clock:
 
 * arrived requests wait until their field fits in the HBM budget (admission by
   free HBM: the budget is a fraction of ``torch.cuda.mem_get_info`` minus what
   admitted requests hold);
 * on every scheduling pass the newly admitted small derivative requests go to
-  the GPU together as one mixed persistent launch (``MixedBatch``); larger
+  the GPU together as one mixed persistent launch, whose job image goes to a
+  reused slot of an ``ImagePool`` (no per-pass allocation or pinning); larger
   derivatives and ADI runs get their own fused launches; every launch group
-  goes to the stream of a pool with the fewest bytes of work in flight, so new work overlaps work
-  already on the device;
+  goes to the stream of a pool with the fewest bytes of work in flight, so new
+  work overlaps work already on the device;
 * completions are observed with events; memory is released and the latency
   (completion - arrival) recorded.
 
@@ -35,11 +41,11 @@ import time
 import numpy as np
 import torch
 
-from .batch import MixedBatch
+from .batch import ImagePool
 from .core import Plan, adi_prepare, adi_steps, compact_derivative
 from . import adi_coeffs, COMPACT_COEFFS
 
-__all__ = ["Request", "make_requests", "Server", "trig_field", "adi_field", "percentile"]
+__all__ = ["Request", "make_requests", "Server", "HostPool", "trig_field", "adi_field", "percentile"]
 
 
 @dataclasses.dataclass
@@ -54,6 +60,7 @@ class Request:
     side: int = 0                  # adi
     steps: int = 0                 # adi
     r: float = 0.0                 # adi
+    upload: bool = False           # input copied from the host (pinned) instead of made on the device
     admitted: float = math.nan
     done: float = math.nan
     result: object = None
@@ -63,16 +70,24 @@ class Request:
         return self.done - self.arrival
 
     def nbytes(self):
+        """Device bytes the request holds while admitted: input and result (ADI:
+        field and work buffer)."""
         if self.kind == "deriv":
             return int(np.prod(self.shape)) * 16
         return self.side * self.side * 16
 
+    def points(self):
+        return int(np.prod(self.shape)) if self.kind == "deriv" else self.side * self.side
+
 
 def make_requests(count, rate, seed=2, deriv_share=0.5, fp32_share=0.5, deriv_sizes=(32, 512),
-                  adi_sides=(256, 2048), adi_steps=(1, 20), r_range=(0.1, 2.0)):
+                  adi_sides=(256, 2048), adi_steps=(1, 20), r_range=(0.1, 2.0), upload_share=0.0):
     """Poisson arrivals at ``rate`` requests/s; per-request kind, sizes, axis,
-    precision and seeds drawn from numpy.random.default_rng(seed)."""
+    precision and seeds drawn from numpy.random.default_rng(seed); a share
+    ``upload_share`` of them (drawn from a second generator, so the rest of
+    the stream does not change) bring their input from the host."""
     rng = np.random.default_rng(seed)
+    up = np.random.default_rng(seed + 7919).random(count) < upload_share
     t = np.cumsum(rng.exponential(1.0 / rate, size=count))
     reqs = []
     for i in range(count):
@@ -86,7 +101,41 @@ def make_requests(count, rate, seed=2, deriv_share=0.5, fp32_share=0.5, deriv_si
                                 side=int(rng.integers(adi_sides[0], adi_sides[1] + 1)),
                                 steps=int(rng.integers(adi_steps[0], adi_steps[1] + 1)),
                                 r=float(rng.uniform(*r_range))))
+        reqs[-1].upload = bool(up[i])
     return reqs
+
+
+class HostPool:
+    """Pinned host memory holding request inputs that cross to the device:
+    ``points`` FP64 values drawn once from numpy.random.default_rng(seed)
+    (normal, sigma 1); a request's input is the first prod(shape) of them,
+    viewed in its shape."""
+
+    def __init__(self, points, seed=11):
+        vals = np.random.default_rng(seed).standard_normal(int(points))
+        self.data = torch.from_numpy(vals).pin_memory()
+        self.points = int(points)
+
+    def view(self, shape):
+        k = int(np.prod(shape))
+        if k > self.points:
+            raise ValueError(f"host pool holds {self.points} points, the request needs {k}")
+        return self.data[:k].view(*shape)
+
+    def grow(self, points, seed=11):
+        if points > self.points:
+            self.__init__(points, seed)
+
+    _shared = None
+
+    @classmethod
+    def shared(cls, points):
+        """One pool per process, grown to the largest request seen (servers share it)."""
+        if cls._shared is None:
+            cls._shared = cls(points)
+        else:
+            cls._shared.grow(points)
+        return cls._shared
 
 
 def trig_field(shape, seed, device):
@@ -125,11 +174,12 @@ class Server:
     """Replays a request stream on one device.
 
     ``streams`` HIP streams (default priority) carry the work; each launch
-    group goes to the stream with the fewest groups in flight.  Derivative
-    requests of at most ``small_points`` points admitted in the same pass are
-    packed into one ``MixedBatch`` launch (there the per-launch cost dominates;
-    a lone small request is a batch of one); larger ones, and ADI runs, get
-    their own fused launches.
+    group goes to the stream with the fewest bytes of work in flight.
+    Derivative requests of at most ``small_points`` points admitted in the same
+    pass are packed into one persistent batch launch (there the per-launch cost
+    dominates; a lone small request is a batch of one), its job image in a
+    reused ``ImagePool`` slot; larger ones, and ADI runs, get their own fused
+    launches.
     """
 
     def __init__(self, device="cuda", hbm_fraction=0.5, hbm_budget=None, max_batch=64, streams=4,
@@ -145,6 +195,11 @@ class Server:
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, int(streams)))]
         self._busy = [0] * len(self.streams)
         self._plans = {}
+        # job-image slots for the batched route: a max_batch image holds at most
+        # max_batch jobs and as many distinct plans (about 2 KB each)
+        self.images = ImagePool(slots=4 * len(self.streams), slot_bytes=self.max_batch * 2304 + 4096,
+                                device=self.device)
+        self.host_pool = None
 
     # -- plan cache (host level tables; shared by all streams) ----------------------
     def plan(self, n, coeffs):
@@ -167,33 +222,41 @@ class Server:
         return i
 
     # -- launching: each returns (event, stream index, [(request, result)], keep-alive) --
+    def _input(self, r, shape):
+        """The request's input on the device, on the current stream: copied from
+        the pinned host pool (upload) or made in place."""
+        if r.upload:
+            t = torch.empty(shape, dtype=torch.float64, device=self.device)
+            t.copy_(self.host_pool.view(shape), non_blocking=True)
+            return t
+        if r.kind == "deriv":
+            return trig_field(shape, r.seed, self.device).contiguous()
+        return adi_field(r.side, r.seed, self.device)
+
     def _launch_derivs(self, reqs, batched, precision=None):
         si = self._stream(sum(r.nbytes() for r in reqs))
         with torch.cuda.stream(self.streams[si]):
             jobs, items = [], []
             for r in reqs:
-                f = trig_field(r.shape, r.seed, self.device).contiguous()
+                f = self._input(r, r.shape)
                 out = torch.empty(r.shape, dtype=torch.float64, device=self.device)
                 jobs.append(dict(f=f, out=out, axis=r.axis, dx=1.0 / (r.shape[r.axis] - 1),
                                  precision=precision or r.precision, sweeps=1))
                 items.append((r, out))
-            keep = jobs
             if batched:
-                batch = MixedBatch(jobs)
-                batch.run()
-                keep = (jobs, batch)
-            else:
-                for j in jobs:
-                    compact_derivative(self.plan(j["f"].shape[j["axis"]], COMPACT_COEFFS), j["f"], j["out"],
-                                       j["dx"], j["axis"], j["precision"], j["sweeps"])
+                keep, ev = self.images.run(jobs)
+                return ev, si, items, keep
+            for j in jobs:
+                compact_derivative(self.plan(j["f"].shape[j["axis"]], COMPACT_COEFFS), j["f"], j["out"],
+                                   j["dx"], j["axis"], j["precision"], j["sweeps"])
             ev = torch.cuda.Event()
             ev.record()
-        return ev, si, items, keep
+        return ev, si, items, jobs
 
     def _launch_adi(self, r, precision=None):
         si = self._stream(r.nbytes() * r.steps)
         with torch.cuda.stream(self.streams[si]):
-            u = adi_field(r.side, r.seed, self.device)
+            u = self._input(r, (r.side, r.side))
             work = torch.empty_like(u)
             adi_prepare(u, work)
             adi_steps(self.plan(r.side, adi_coeffs(r.r)), u, work, r.r, r.steps, precision or r.precision, 1)
@@ -213,8 +276,10 @@ class Server:
                 reqs.append(Request(len(reqs), "deriv", 0.0, prec, 1, shape=(32, 32, 32), axis=axis))
                 reqs.append(Request(len(reqs), "deriv", 0.0, prec, 2, shape=(side, side, side), axis=axis))
             reqs.append(Request(len(reqs), "adi", 0.0, prec, 3, side=256, steps=2, r=0.5))
+        up = [Request(len(reqs) + i, q.kind, 0.0, q.precision, q.seed, shape=q.shape, axis=q.axis,
+                      side=q.side, steps=q.steps, r=q.r, upload=True) for i, q in enumerate(reqs)]
         for mode in ("continuous", "sequential"):
-            self.run(reqs, mode=mode)
+            self.run(reqs + up, mode=mode)
 
     # -- the scheduling loop ------------------------------------------------------------
     def run(self, requests, mode="continuous", time_scale=1.0):
@@ -226,6 +291,9 @@ class Server:
         for q in reqs:
             q.done = q.admitted = math.nan
             q.result = None
+        need = max([q.points() for q in reqs if q.upload] or [0])
+        if need:                  # pinned host inputs exist before the clock starts
+            self.host_pool = HostPool.shared(need)
         pending = collections.deque(reqs)
         waiting = collections.deque()
         inflight = []              # (event, stream index, [(request, result tensor)], keep-alive)
@@ -290,7 +358,8 @@ class Server:
         torch.cuda.synchronize(self.device)
         lat = [q.latency for q in reqs]
         span = max(q.done for q in reqs) - min(q.arrival * time_scale for q in reqs)
-        return {"mode": mode, "requests": len(reqs), "p50_ms": 1e3 * percentile(lat, 50),
+        return {"mode": mode, "requests": len(reqs), "uploads": sum(1 for q in reqs if q.upload),
+                "p50_ms": 1e3 * percentile(lat, 50),
                 "p99_ms": 1e3 * percentile(lat, 99), "mean_ms": 1e3 * float(np.mean(lat)),
                 "throughput_rps": len(reqs) / span if span > 0 else math.inf, "batches": batches,
                 "streams": len(self.streams), "budget_bytes": self.budget}
