@@ -304,9 +304,9 @@ def adi_case(N, r, nsteps, reps):
     s1.step(u1, 1)
     want = oadi.pr_run(u0, r, 1)
     err1 = float(np.abs(u1.cpu().numpy() - want).max() / np.abs(want).max())
+    # the 32 MiB field stays in the 256 MB Infinity Cache: effective, not HBM, GB/s
     res = {"N": N, "r": r, "steps": nsteps, "ms_per_step": 1e3 * t / nsteps,
-           "effective_gbps": 2 * nsteps * half_bytes / t / 1e9, "err_vs_oracle_1step": err1,
-           "note": "32 MiB field stays in the 256 MB Infinity Cache: effective, not HBM, GB/s"}
+           "effective_gbps": 2 * nsteps * half_bytes / t / 1e9, "err_vs_oracle_1step": err1}
     from ntamd import adi_coeffs
     from ntamd.partition import PartitionPlan
     pp = PartitionPlan(N, 16, adi_coeffs(r))
@@ -419,7 +419,6 @@ def config4_case(N=1024, reps=5):
             f"err vs oracle {er:.2e} / {e64:.2e} / {es:.2e}")
     del f32, f64, out
     torch.cuda.empty_cache()
-    res["err_sample"] = "256 lines per axis and form"
     return res
 
 
@@ -514,8 +513,8 @@ def served_summary(serving):
     batching beside one-at-a-time plain FP64 (None when the case did not run)."""
     if not serving:
         return None
+    # rate_*: inputs resident on the device; upload_rate_*: every input copied from pinned host memory
     out = {"requests": serving["requests"], "arrivals": "Poisson, numpy default_rng(2)"}
-    out["uploads"] = "upload_rate_*: every request's input copied host->device (pinned) on its stream"
     for key, case in serving.items():
         if key.startswith("rate_") or key.startswith("upload_rate_"):
             out[key] = {m: {k: case[m][k] for k in ("p50_ms", "p99_ms", "throughput_rps")}
