@@ -86,3 +86,43 @@ def test_batch_rejects_bad_jobs(dev):
         MixedBatch([dict(f=g, out=torch.empty_like(g), axis=1, dx=0.1)])   # n = 8 < 16
     with pytest.raises(NtInvalidArgument):
         MixedBatch([dict(f=f, out=f, axis=1, dx=0.1)])                      # aliasing
+
+
+def test_batch_every_job_class(dev):
+    """One launch holding a job of every class the batch kernel specialises:
+    x, y and z lines; 1, 2, 4 and 8 rows per lane, each with a power-of-two
+    length (compile-time level structure) and a general one; FP64 and FP32 +
+    refinement; in-place solves, compact derivatives of FP64 fields and of
+    FP32-stored fields (FP32 rows at one row per lane included).  Strided
+    fields with 16-line-aligned slabs take the 16-byte tile path, the others
+    (37 lines) the element path.  Every job equals the oracle."""
+    from ntamd.batch import MixedBatch
+    co = (2.0, 0.3, 0.5, 3.0, 0.7, 0.2, 1.7)
+    lengths = [16, 37, 64, 100, 128, 200, 256, 300, 512]     # R = 1, 1, 1(full), 2, 2(full), 4, 4(full), 8, 8(full)
+    jobs, want = [], []
+    rng = np.random.default_rng(21)
+    for axis in (2, 1, 0):
+        for n in lengths:
+            for prec in ("fp64", "fp32_refine"):
+                for kind in ("solve", "cd64", "cd32"):
+                    other = 37 if (n + len(jobs)) % 2 else 32
+                    shape = {2: (2, other, n), 1: (2, n, other), 0: (n, 1, other)}[axis]
+                    f = rng.standard_normal(shape)
+                    if kind == "cd32":
+                        f = f.astype(np.float32).astype(np.float64)
+                    h = 0.05 + 0.001 * len(jobs)
+                    if kind == "solve":
+                        x = torch.tensor(f, device=dev)
+                        jobs.append(dict(out=x, axis=axis, solve=True, coeffs=co, precision=prec, sweeps=2))
+                        ref = compact.last_to_axis(cr.banded_solve(co, compact.axis_to_last(f, axis)), axis)
+                    else:
+                        ft = torch.tensor(f.astype(np.float32) if kind == "cd32" else f, device=dev)
+                        jobs.append(dict(f=ft, out=torch.full(shape, float("nan"), dtype=torch.float64, device=dev),
+                                         axis=axis, dx=h, precision=prec, sweeps=2))
+                        ref = compact.derivative_banded(f, h, axis)
+                    want.append((ref, prec, (axis, n, prec, kind)))
+    MixedBatch(jobs).run()
+    torch.cuda.synchronize()
+    for j, (ref, prec, tag) in zip(jobs, want):
+        tol = 1e-12 if prec == "fp64" else 1e-10
+        assert rel_err(j["out"].cpu().numpy(), ref) < tol, tag
