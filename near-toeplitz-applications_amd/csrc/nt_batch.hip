@@ -377,11 +377,11 @@ __device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t
   };
 
   TIn pf[PF];
-  auto fetch = [&](const Tile& g, int i0, int i1) {     // vectors i0 .. i1-1 (the scalar path: all)
+  auto fetch = [&](const Tile& g) {
     const TIn* __restrict__ src = in + g.base;
     if (g.vec) {
 #pragma unroll
-      for (int i = i0; i < i1; ++i) {
+      for (int i = 0; i < PFV; ++i) {
         const int idx = tid + i * BATCH_NT;
         const int vr = idx % VPR;
         const int el = CLAMP_V ? min(idx / VPR, n - 1) : idx / VPR;
@@ -398,7 +398,7 @@ __device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t
           pf[4 * i + 3] = v.w;
         }
       }
-    } else if (i0 == 0) {
+    } else {
 #pragma unroll
       for (int i = 0; i < PFS; ++i) {
         const int idx = tid + i * BATCH_NT;
@@ -407,9 +407,6 @@ __device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t
       }
     }
   };
-  // Vectors of the next tile fetched before the solve (all of them); the rest
-  // would follow it.
-  constexpr int PF_EARLY = PFV;
   auto deposit = [&](const Tile& g) {
     if (g.vec) {
 #pragma unroll
@@ -481,7 +478,7 @@ __device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t
 
   int64_t t = t0;
   Tile g = geometry(t);
-  fetch(g, 0, PFV);
+  fetch(g);
   for (;;) {
     tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (BATCH_NT - 1);   // the mask keeps its range known
     __syncthreads();                             // the LDS is free (previous tile stored, previous chunk done)
@@ -490,12 +487,11 @@ __device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t
     Tile gn = g;
     if (more) {
       gn = geometry(t + 1);
-      fetch(gn, 0, PF_EARLY);                    // the next tile's loads fly during this one
+      fetch(gn);                                 // the next tile's loads fly during this one
     }
     __syncthreads();
     solve_tile(g);
     __syncthreads();
-    if (PF_EARLY < PFV && more) fetch(gn, PF_EARLY, PFV);
     // the store offsets from a fresh opaque index: not computed before the
     // solve and held through it
     tid = (static_cast<int>(threadIdx.x) + opaque_vzero()) & (BATCH_NT - 1);
@@ -567,11 +563,7 @@ batch_kernel(const BatchArgs args) {
     const int64_t t1e = t0 + J.chunk_tiles;
     const int64_t t1 = t1e < J.ntiles ? t1e : J.ntiles;
     const NtPlanK& P = *(const NtPlanK*)(&plans[J.plan]);
-#ifdef NT_BATCH_CLASS_LO
-    dispatch_class<NT_BATCH_CLASS_LO, NT_BATCH_CLASS_HI>(J.cls, J, P, t0, t1, lds);
-#else
     dispatch_class<0, NUM_CLASSES>(J.cls, J, P, t0, t1, lds);
-#endif
     if (threadIdx.x == 0) s_chunk[slot ^ 1] = nxt;
     __syncthreads();
     slot ^= 1;
