@@ -364,7 +364,16 @@ int nt_part_solve_axis(const nt_part_plan* pp, int producer, const void* in, voi
     return nt::set_error(NT_EUNSUPPORTED, "%s", "nt_part_solve_axis: tile does not fit in LDS");
   a.tl_log2 = tls;
   a.lp = lp;
-  a.tiles = (nsys + TL - 1) / TL;
+  // pairs of adjacent strided lines as 16-byte vectors: every row and slab start,
+  // and both buffers, 16-byte aligned (tiles start at even lines)
+  a.vec = (layout == nt::STRIDED && a.estride % 2 == 0 && a.ostride % 2 == 0 &&
+           reinterpret_cast<uintptr_t>(in) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0)
+              ? 1
+              : 0;
+  // paired strided ADI tiles start at line 0 (see part_tile_kernel): lines 1 .. phys - 2
+  // in tiles of TL from line 0; other tiles start at the first solved line
+  const bool colal = (producer == 2 && layout == nt::STRIDED && (NT_PART_VEC & 1) && a.vec);
+  a.tiles = colal ? (a.phys - 2) / TL + 1 : (nsys + TL - 1) / TL;
   const int64_t grid = outer * a.tiles;
   if (grid > 0x7fffffffLL) return nt::set_error(NT_EINVAL, "%s", "nt_part_solve_axis: too many lines");
   const int S = 64 / W;

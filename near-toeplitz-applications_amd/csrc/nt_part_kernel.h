@@ -254,6 +254,8 @@ struct PartTileArgs {
   double s1, s2;        // producer scalars as in LineArgs
   int32_t tl_log2;      // log2 of the lines per tile
   int32_t lp;           // LDS doubles per staged line
+  int32_t vec;          // strided: pairs of adjacent lines move as 16-byte vectors
+  int32_t pad_;
 };
 
 template <int R>
@@ -275,6 +277,11 @@ __device__ __forceinline__ double part_rhs(const PartTileArgs& args, int e, int 
   }
   return c;
 }
+
+// Strided tiles with 16-byte aligned rows: 1 pairs the staging loads, 2 the stores.
+#ifndef NT_PART_VEC
+#define NT_PART_VEC 3
+#endif
 
 template <int R, int W, int PROD, int LAYOUT, bool FULL>
 __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args) {
@@ -303,15 +310,22 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
   const int sl = lane & (W - 1);
   auto at = [&](int l, int e) -> int { return l * LP + e + (e >> LR) * CP; };
 
-  // tile geometry: solved lines p0 .. p0 + rows - 1 of slab o (staged index + HALO)
-  const int64_t nsys = args.phys - 2 * HALO;
+  // tile geometry: staged lines lo .. hi - 1 of the tile are the slab's lines
+  // c0 + lo .. c0 + hi - 1.  Paired strided ADI tiles (args.vec) start at line 0,
+  // so their rows are 16-byte aligned, and leave the boundary lines 0 and phys - 1
+  // out (lo = 1 in the first tile, hi = phys - 1 - c0 in the last); other tiles
+  // start at the first solved line.
+  const bool colal = (HALO != 0 && LAYOUT == STRIDED && (NT_PART_VEC & 1) && args.vec);
   const int64_t t = blockIdx.x;
   const int64_t o = t / args.tiles;
   const int64_t p0 = (t - o * args.tiles) << TLS;
-  const int64_t rv = nsys - p0;
-  const int rows = rv < TL ? static_cast<int>(rv) : TL;
+  const int64_t c0 = colal ? p0 : p0 + HALO;
+  const int lo = (colal && p0 == 0) ? 1 : 0;
+  const int64_t rv = (colal ? args.phys - 1 : args.phys - HALO) - c0;
+  const int hi = rv < TL ? static_cast<int>(rv) : TL;
+  auto valid = [&](int l) -> bool { return l >= lo && l < hi; };
   const int64_t ls = args.lstride, es = args.estride;
-  const int64_t base = o * args.ostride + (p0 + HALO) * ls;
+  const int64_t base = o * args.ostride + c0 * ls;
   const double* __restrict__ in = static_cast<const double*>(args.in) + base;
   double* __restrict__ out = static_cast<double*>(args.out) + base;
 
@@ -327,7 +341,59 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
   // Loads for PU points per thread are issued before any of them is used.
   constexpr int PU = 8;
   const int total = n << TLS;
-  if (LAYOUT == STRIDED) {
+  if (LAYOUT == STRIDED && (NT_PART_VEC & 1) && args.vec) {
+    // two adjacent lines l, l + 1 (l even) of a row per thread, TL / 2 consecutive
+    // threads per row: their points are one 16-byte aligned vector (c0 is even).
+    // ADI takes the outer neighbours, lines l - 1 and l + 2, as scalars; the pair
+    // is loaded when either line is solved (both are lines of the slab).
+    constexpr int PV = PU / 2;                   // pairs per thread: the same points in flight
+    const int total2 = total >> 1;
+    const int hm = (TL >> 1) - 1;
+    for (int i0 = tid; i0 < total2; i0 += PV * nt) {
+      nt_dbl2 c[PV], a[PV], b[PV];
+#pragma unroll
+      for (int u = 0; u < PV; ++u) {
+        const int idx = i0 + u * nt;
+        const int l = (idx & hm) << 1, e = idx >> (TLS - 1);
+        const bool v0 = idx < total2 && valid(l), v1 = idx < total2 && valid(l + 1);
+        c[u].x = c[u].y = a[u].x = a[u].y = b[u].x = b[u].y = 0.0;
+        const double* p = in + static_cast<int64_t>(e) * es + l;
+        if (PROD == P_ADI) {
+          if (v0 || v1) c[u] = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(p));
+          if (v0) a[u].x = p[-1];
+          if (v1) b[u].x = p[2];
+        } else if (v0 && v1) {
+          c[u] = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(p));
+          if (PROD != P_IDENT) {
+            a[u] = *reinterpret_cast<const nt_dbl2*>(p + nb(e, 0));
+            b[u] = *reinterpret_cast<const nt_dbl2*>(p + nb(e, 1));
+          }
+        } else if (v0) {
+          c[u].x = p[0];
+          if (PROD != P_IDENT) {
+            a[u].x = p[nb(e, 0)];
+            b[u].x = p[nb(e, 1)];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PV; ++u) {
+        const int idx = i0 + u * nt;
+        const int l = (idx & hm) << 1, e = idx >> (TLS - 1);
+        if (idx >= total2) continue;
+        double r0, r1;
+        if (PROD == P_ADI) {
+          r0 = part_rhs<PROD>(args, e, n, a[u].x, c[u].x, c[u].y);
+          r1 = part_rhs<PROD>(args, e, n, c[u].x, c[u].y, b[u].x);
+        } else {
+          r0 = part_rhs<PROD>(args, e, n, a[u].x, c[u].x, b[u].x);
+          r1 = part_rhs<PROD>(args, e, n, a[u].y, c[u].y, b[u].y);
+        }
+        if (valid(l)) lds[at(l, e)] = r0;
+        if (valid(l + 1)) lds[at(l + 1, e)] = r1;
+      }
+    }
+  } else if (LAYOUT == STRIDED) {
     // TL points of a row per TL consecutive threads
     for (int i0 = tid; i0 < total; i0 += PU * nt) {
       double c[PU], a[PU], b[PU];
@@ -336,7 +402,7 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
         const int idx = i0 + u * nt;
         const int l = idx & (TL - 1), e = idx >> TLS;
         c[u] = a[u] = b[u] = 0.0;
-        if (idx < total && l < rows) {
+        if (idx < total && valid(l)) {
           const double* p = in + static_cast<int64_t>(e) * es + l;
           c[u] = __builtin_nontemporal_load(p);
           if (PROD != P_IDENT) {
@@ -349,7 +415,7 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
       for (int u = 0; u < PU; ++u) {
         const int idx = i0 + u * nt;
         const int l = idx & (TL - 1), e = idx >> TLS;
-        if (idx < total && l < rows) lds[at(l, e)] = part_rhs<PROD>(args, e, n, a[u], c[u], b[u]);
+        if (idx < total && valid(l)) lds[at(l, e)] = part_rhs<PROD>(args, e, n, a[u], c[u], b[u]);
       }
     }
   } else {
@@ -366,7 +432,7 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
         lu[u] = l;
         eu[u] = e;
         c[u] = a[u] = b[u] = 0.0;
-        if (l < rows) {
+        if (valid(l)) {
           const double* p = in + l * ls + e;
           c[u] = __builtin_nontemporal_load(p);
           if (PROD != P_IDENT) {
@@ -385,7 +451,7 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
       e0 = e;
 #pragma unroll
       for (int u = 0; u < PU; ++u)
-        if (lu[u] < rows) lds[at(lu[u], eu[u])] = part_rhs<PROD>(args, eu[u], n, a[u], c[u], b[u]);
+        if (valid(lu[u])) lds[at(lu[u], eu[u])] = part_rhs<PROD>(args, eu[u], n, a[u], c[u], b[u]);
     }
   }
   __syncthreads();
@@ -398,7 +464,7 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
     const int q = tau0 >> TLS;                   // the same for the S segments
     const int l = (tau0 & (TL - 1)) + seg;
     const int kind = (P == 1) ? 0 : (q == 0 ? 0 : (q == P - 1 ? 2 : 1));
-    const bool active = l < rows;
+    const bool active = valid(l);
     const int e0 = q * m + sl * R;
     double d[R];
 #pragma unroll
@@ -420,7 +486,7 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
   for (int i = tid; i < (NP2 << TLS); i += nt) {
     const int l = i / NP2, r = i - l * NP2;
     double u = 0.0;
-    if (l < rows) {
+    if (valid(l)) {
       const double* __restrict__ row = D.red_inv + r * NP2;
       const double* __restrict__ f = red + l * NP2;
       for (int j = 0; j < NP2; ++j) u += row[j] * f[j];
@@ -435,7 +501,7 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
     const int q = tau0 >> TLS;
     const int l = (tau0 & (TL - 1)) + seg;
     const int kind = (P == 1) ? 0 : (q == 0 ? 0 : (q == P - 1 ? 2 : 1));
-    if (l < rows) {
+    if (valid(l)) {
       const double alpha = sol[l * NP2 + 2 * q], beta = sol[l * NP2 + 2 * q + 1];
       const double* __restrict__ xu = args.xu + kind * m + sl * R;
       const double* __restrict__ xl = args.xl + kind * m + sl * R;
@@ -448,13 +514,32 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
   __syncthreads();
 
   // ---- store ---------------------------------------------------------------------------
-  if (LAYOUT == STRIDED) {
+  if (LAYOUT == STRIDED && (NT_PART_VEC & 2) && args.vec && (HALO == 0 || colal)) {
+    // lines l, l + 1 (l even) of a row as one aligned 16-byte store (c0 is even),
+    // TL / 2 threads per row; a pair with one solved line stores that line alone
+    const int hm = (TL >> 1) - 1;
+    for (int idx = tid; idx < (total >> 1); idx += nt) {
+      const int l = (idx & hm) << 1, e = idx >> (TLS - 1);
+      double* __restrict__ row = out + static_cast<int64_t>(e) * es;
+      const bool v0 = valid(l), v1 = valid(l + 1);
+      if (v0 && v1) {
+        nt_dbl2 v;
+        v.x = lds[at(l, e)];
+        v.y = lds[at(l + 1, e)];
+        __builtin_nontemporal_store(v, reinterpret_cast<nt_dbl2*>(row + l));
+      } else if (v0) {
+        __builtin_nontemporal_store(lds[at(l, e)], row + l);
+      } else if (v1) {
+        __builtin_nontemporal_store(lds[at(l + 1, e)], row + l + 1);
+      }
+    }
+  } else if (LAYOUT == STRIDED) {
     for (int idx = tid; idx < total; idx += nt) {
       const int l = idx & (TL - 1), e = idx >> TLS;
-      if (l < rows) __builtin_nontemporal_store(lds[at(l, e)], out + static_cast<int64_t>(e) * es + l);
+      if (valid(l)) __builtin_nontemporal_store(lds[at(l, e)], out + static_cast<int64_t>(e) * es + l);
     }
   } else {
-    for (int l = 0; l < rows; ++l) {
+    for (int l = lo; l < hi; ++l) {
       double* __restrict__ dst = out + l * ls;
       for (int e = tid; e < n; e += nt) __builtin_nontemporal_store(lds[at(l, e)], dst + e);
     }

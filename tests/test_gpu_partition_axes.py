@@ -49,6 +49,33 @@ def test_partitioned_compact_tiles_equal_oracle(dev, n, P, axis):
     assert rel_err(out.cpu().numpy(), compact.derivative_banded(f, h, axis)) < 1e-12
 
 
+@pytest.mark.parametrize("n,P", [(512, 4), (2048, 16), (96, 3), (64, 16), (1024, 8), (2048, 8), (34, 2)])
+@pytest.mark.parametrize("axis", [1, 0])
+def test_partitioned_compact_paired_strided_equals_oracle(dev, n, P, axis):
+    """Even row strides: adjacent lines move in 16-byte pairs. 22 and 46 lines
+    leave ragged (even) last tiles. The same fields in views 8 bytes into their
+    buffers take the unpaired path."""
+    from ntamd.partition import PartitionPlan
+    from ntamd.workloads import compact_field
+    shape = {1: (3, n, 22), 0: (n, 2, 23)}[axis]
+    f = compact_field(shape, seed=3 * n + P + axis)
+    h = 1.0 / (n - 1)
+    plan = PartitionPlan(n, P, COMPACT)
+    want = compact.derivative_banded(f, h, axis)
+    fd = torch.tensor(f, device=dev)
+    out = torch.full_like(fd, float("nan"))
+    plan.compact_derivative(fd, out, h, axis)
+    assert rel_err(out.cpu().numpy(), want) < 1e-12
+    # 8-byte offset views: the unpaired path
+    fb = torch.zeros(fd.numel() + 1, dtype=torch.float64, device=dev)
+    ob = torch.full_like(fb, float("nan"))
+    fv = fb[1:].view(shape)
+    ov = ob[1:].view(shape)
+    fv.copy_(fd)
+    plan.compact_derivative(fv, ov, h, axis)
+    assert rel_err(ov.cpu().numpy(), want) < 1e-12
+
+
 @pytest.mark.parametrize("n,P", [(512, 4), (2048, 16), (100, 2), (1024, 8)])
 @pytest.mark.parametrize("axis", [2, 1, 0])
 @pytest.mark.parametrize("coeffs", [COMPACT, GENERAL])
@@ -159,10 +186,11 @@ def test_dropin_reference_sequence_y_and_z(dev, N, P, direction):
     assert rel_err(got, want) < 1e-12
 
 
-@pytest.mark.parametrize("N,P,r", [(2048, 16, 0.5), (512, 4, 1.3), (130, 2, 0.2), (1024, 8, 2.0)])
+@pytest.mark.parametrize("N,P,r", [(2048, 16, 0.5), (512, 4, 1.3), (130, 2, 0.2), (1024, 8, 2.0), (99, 3, 0.8)])
 def test_partitioned_adi_steps_equal_oracle(dev, N, P, r):
     """Both half steps partitioned (x-implicit rows, y-implicit columns), two
-    steps from the Gaussian initial condition, against the oracle's PR steps."""
+    steps from the Gaussian initial condition, against the oracle's PR steps.
+    Even N moves the columns in 16-byte pairs, odd N (99) one at a time."""
     from ntamd.partition import PartitionedADISolver
     u0 = adi.gaussian_ic(N, seed=N)
     u = torch.tensor(u0, device=dev)
