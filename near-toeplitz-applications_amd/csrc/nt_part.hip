@@ -366,8 +366,12 @@ int nt_part_solve_axis(const nt_part_plan* pp, int producer, const void* in, voi
   a.lp = lp;
   // pairs of adjacent strided lines as 16-byte vectors: every row and slab start,
   // and both buffers, 16-byte aligned (tiles start at even lines)
-  a.vec = (layout == nt::STRIDED && a.estride % 2 == 0 && a.ostride % 2 == 0 &&
-           reinterpret_cast<uintptr_t>(in) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0)
+  // contiguous lines: pairs of adjacent points, with n even and every line start
+  // 16-byte aligned
+  const bool al16 = reinterpret_cast<uintptr_t>(in) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0 &&
+                    a.ostride % 2 == 0;
+  a.vec = ((layout == nt::STRIDED && al16 && a.estride % 2 == 0) ||
+           (layout == nt::CONTIG && al16 && pp->n % 2 == 0 && pp->n >= 4 && a.lstride % 2 == 0))
               ? 1
               : 0;
   // paired strided ADI tiles start at line 0 (see part_tile_kernel): lines 1 .. phys - 2

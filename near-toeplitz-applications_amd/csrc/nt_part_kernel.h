@@ -418,6 +418,68 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
         if (idx < total && valid(l)) lds[at(l, e)] = part_rhs<PROD>(args, e, n, a[u], c[u], b[u]);
       }
     }
+  } else if ((NT_PART_VEC & 1) && args.vec) {
+    // contiguous lines, points e, e + 1 (e even) of a line per thread as one
+    // 16-byte vector; ADI's neighbours (the adjacent lines) are vectors too, a
+    // compact pair's outer neighbours e - 1 and e + 2 scalars (n is even, >= 4)
+    constexpr int PV = PU / 2;
+    const int n2 = n >> 1;
+    const int total2 = total >> 1;
+    const int step_l = nt / n2, step_e = nt - step_l * n2;
+    int l0 = tid / n2, e0 = tid - l0 * n2;     // e in pairs
+    for (int i0 = tid; i0 < total2; i0 += PV * nt) {
+      nt_dbl2 c[PV], a[PV], b[PV];
+      int lu[PV], eu[PV];
+      int l = l0, e = e0;
+#pragma unroll
+      for (int u = 0; u < PV; ++u) {
+        lu[u] = l;
+        eu[u] = 2 * e;
+        c[u].x = c[u].y = a[u].x = a[u].y = b[u].x = b[u].y = 0.0;
+        if (valid(l)) {
+          const int ee = 2 * e;
+          const double* p = in + l * ls + ee;
+          c[u] = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(p));
+          if (PROD == P_ADI) {
+            a[u] = *reinterpret_cast<const nt_dbl2*>(p - ls);
+            b[u] = *reinterpret_cast<const nt_dbl2*>(p + ls);
+          } else if (PROD == P_COMPACT) {
+            if (ee > 0) a[u].x = p[-1];
+            if (ee + 2 < n) b[u].x = p[2];
+          }
+        }
+        l += step_l;
+        e += step_e;
+        if (e >= n2) {
+          e -= n2;
+          ++l;
+        }
+      }
+      l0 = l;
+      e0 = e;
+#pragma unroll
+      for (int u = 0; u < PV; ++u) {
+        if (!valid(lu[u])) continue;
+        const int ee = eu[u];
+        double r0, r1;
+        if (PROD == P_ADI) {
+          r0 = part_rhs<PROD>(args, ee, n, a[u].x, c[u].x, b[u].x);
+          r1 = part_rhs<PROD>(args, ee + 1, n, a[u].y, c[u].y, b[u].y);
+        } else if (PROD == P_COMPACT) {
+          // neighbours as nb(): closures take f[1], f[2] at e = 0 and f[n-2], f[n-3] at n - 1
+          const double left = a[u].x, right = b[u].x;
+          r0 = (ee == 0) ? part_rhs<PROD>(args, 0, n, c[u].y, c[u].x, right)
+                         : part_rhs<PROD>(args, ee, n, left, c[u].x, c[u].y);
+          r1 = (ee + 1 == n - 1) ? part_rhs<PROD>(args, n - 1, n, c[u].x, c[u].y, left)
+                                 : part_rhs<PROD>(args, ee + 1, n, c[u].x, c[u].y, right);
+        } else {
+          r0 = c[u].x;
+          r1 = c[u].y;
+        }
+        lds[at(lu[u], ee)] = r0;
+        lds[at(lu[u], ee + 1)] = r1;
+      }
+    }
   } else {
     // consecutive threads along each line; point idx of the tile is (idx / n, idx % n),
     // advanced incrementally from one division per thread
@@ -537,6 +599,16 @@ __global__ void __launch_bounds__(1024) part_tile_kernel(const PartTileArgs args
     for (int idx = tid; idx < total; idx += nt) {
       const int l = idx & (TL - 1), e = idx >> TLS;
       if (valid(l)) __builtin_nontemporal_store(lds[at(l, e)], out + static_cast<int64_t>(e) * es + l);
+    }
+  } else if ((NT_PART_VEC & 2) && args.vec) {
+    for (int l = lo; l < hi; ++l) {
+      double* __restrict__ dst = out + l * ls;
+      for (int e = 2 * tid; e < n; e += 2 * nt) {
+        nt_dbl2 v;
+        v.x = lds[at(l, e)];
+        v.y = lds[at(l, e + 1)];
+        __builtin_nontemporal_store(v, reinterpret_cast<nt_dbl2*>(dst + e));
+      }
     }
   } else {
     for (int l = lo; l < hi; ++l) {

@@ -50,14 +50,14 @@ def test_partitioned_compact_tiles_equal_oracle(dev, n, P, axis):
 
 
 @pytest.mark.parametrize("n,P", [(512, 4), (2048, 16), (96, 3), (64, 16), (1024, 8), (2048, 8), (34, 2)])
-@pytest.mark.parametrize("axis", [1, 0])
-def test_partitioned_compact_paired_strided_equals_oracle(dev, n, P, axis):
-    """Even row strides: adjacent lines move in 16-byte pairs. 22 and 46 lines
-    leave ragged (even) last tiles. The same fields in views 8 bytes into their
-    buffers take the unpaired path."""
+@pytest.mark.parametrize("axis", [2, 1, 0])
+def test_partitioned_compact_paired_equals_oracle(dev, n, P, axis):
+    """16-byte pairs: adjacent points of x-lines, adjacent y- or z-lines (even
+    row strides; 22 and 46 lines leave ragged, even last tiles). The same fields
+    in views 8 bytes into their buffers take the unpaired path."""
     from ntamd.partition import PartitionPlan
     from ntamd.workloads import compact_field
-    shape = {1: (3, n, 22), 0: (n, 2, 23)}[axis]
+    shape = {2: (3, 7, n), 1: (3, n, 22), 0: (n, 2, 23)}[axis]
     f = compact_field(shape, seed=3 * n + P + axis)
     h = 1.0 / (n - 1)
     plan = PartitionPlan(n, P, COMPACT)
