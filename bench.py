@@ -370,16 +370,18 @@ def measured_traffic():
 
 
 def pmc_probe():
-    """Three headline launches, then three copies of the same 1 GiB field: the
-    program profiled by the FETCH_SIZE / WRITE_SIZE passes."""
+    """Three launches each of the headline d/dx, then d/dy and d/dz, then three
+    copies of the same 1 GiB field: the program profiled by the FETCH_SIZE /
+    WRITE_SIZE passes."""
     import torch
     from ntamd import COMPACT_COEFFS, Plan, core
     N = 512
     f = torch.randn((N, N, N), dtype=torch.float64, device="cuda")
     out = torch.empty_like(f)
     plan = Plan(N, COMPACT_COEFFS)
-    for _ in range(3):
-        core.compact_derivative(plan, f, out, 1.0 / (N - 1), 2)
+    for axis in (2, 1, 0):
+        for _ in range(3):
+            core.compact_derivative(plan, f, out, 1.0 / (N - 1), axis)
     for _ in range(3):
         out.copy_(f)
     torch.cuda.synchronize()
