@@ -166,6 +166,23 @@ def adi_field(side, seed, device):
     return u.contiguous()
 
 
+def admit(waiting, used, budget, max_batch):
+    """Admission by free HBM, first come first served: pop requests from the
+    front of ``waiting`` while their fields fit in ``budget`` bytes beside the
+    ``used`` bytes of requests in flight, at most ``max_batch`` of them.  The
+    head of the queue blocks those behind it (no overtaking).  A request larger
+    than the whole budget is admitted alone once nothing else is in flight.
+    Returns (admitted requests, new used bytes)."""
+    admitted = []
+    while waiting and len(admitted) < max_batch:
+        need = waiting[0].nbytes()
+        if used + need > budget and (used > 0 or admitted):
+            break                       # wait for memory to free up
+        admitted.append(waiting.popleft())
+        used += need
+    return admitted, used
+
+
 def percentile(values, q):
     return float(np.percentile(np.asarray(values, dtype=np.float64), q))
 
@@ -323,16 +340,10 @@ class Server:
                     still.append((ev, si, items, keep))
             inflight = still
             if mode == "continuous":
-                admitted = []
-                while waiting and len(admitted) < self.max_batch:
-                    q = waiting[0]
-                    need = q.nbytes()
-                    if used + need > self.budget and (used > 0 or admitted):
-                        break                       # wait for memory to free up
-                    waiting.popleft()
-                    used += need
-                    q.admitted = clock()
-                    admitted.append(q)
+                admitted, used = admit(waiting, used, self.budget, self.max_batch)
+                t_adm = clock()
+                for q in admitted:
+                    q.admitted = t_adm
                 small = [q for q in admitted if self._batchable(q)]
                 for q in admitted:
                     if q.kind == "adi":
