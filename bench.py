@@ -525,6 +525,54 @@ def served_summary(serving):
     return out
 
 
+def compact_extras(extras):
+    """The extra cases as short arrays for the stdout line (the full record,
+    with every field named, goes to stderr).  The legend travels in the line."""
+    out = {}
+    legend = {}
+    us_err = {k: [v["us"], v["err_vs_oracle"]] for k, v in extras.items()
+              if isinstance(v, dict) and "us" in v}
+    if us_err:
+        out["us_err"] = us_err
+        legend["us_err"] = "512^3 field of the headline: [us per launch, max rel err vs oracle on 256 lines]"
+    a = extras.get("adi_2048_200steps_graph")
+    if a:
+        out["adi_2048"] = [a["ms_per_step"], a["err_vs_oracle_1step"], a["partitioned_ms_per_step"],
+                           a["partitioned_x_half_step_16x128_us"], a["partitioned_y_half_step_16x128_us"],
+                           a["partitioned_err_vs_oracle_1step"]]
+        legend["adi_2048"] = ("200 steps in one hipGraph: [ms/step, err after 1 step; partitioned 16x128 + "
+                              "reduced 32: ms/step, x half step us, y half step us, err after 1 step]")
+    c = extras.get("fp32_storage_1024")
+    if c:
+        out["fp32_storage_1024"] = {ax: [r["ms"]["fp32_refine"], r["ms"]["fp64"], r["ms"]["f32_store_fp64"],
+                                         r["speedup_vs_fp64"], max(r["err"].values())]
+                                    for ax, r in c.items()}
+        legend["fp32_storage_1024"] = ("[ms FP32 storage + FP32 solve + 1 sweep, ms plain FP64, ms FP32 storage "
+                                       "+ FP64 solve, speedup of the first over plain FP64, worst err vs oracle]")
+    for key in ("mixed_batch_64", "mixed_batch_small_1024"):
+        b = extras.get(key)
+        if b:
+            out[key] = [b["grids"], b["launch_ms"], b["separate_launches_ms"], b["err_vs_oracle_per_grid"]["max"]]
+            legend["mixed_batch"] = "[grids, ms one persistent launch, ms separate launches, worst grid err vs oracle]"
+    if out:
+        out["legend"] = legend
+    return out
+
+
+def compact_served(served):
+    """p50/p99 ms and requests/s per arrival rate: continuous beside sequential."""
+    if not served:
+        return None
+    out = {"requests": served["requests"],
+           "fields": "[p50_ms, p99_ms, req/s] continuous | sequential plain FP64; upload_* = inputs "
+                     "copied from pinned host memory"}
+    for key, case in served.items():
+        if key.startswith("rate_") or key.startswith("upload_rate_"):
+            out[key] = {m: [case[m]["p50_ms"], case[m]["p99_ms"], case[m]["throughput_rps"]]
+                        for m in ("continuous", "sequential")}
+    return out
+
+
 EXTRA_CASES = ("axes", "refine", "adi", "fp32_storage", "batch", "serving")
 
 
@@ -651,7 +699,18 @@ def main():
         rec["tuning"] = tuned
     full = json.dumps(rec)
     log(f"[bench] detail {full}")
-    small = json.dumps(sig(rec), separators=(",", ":"))
+    # The stdout line: the same record with the extra cases, the served stream and
+    # the CPU baseline in short form (about 2.5 KB).
+    line_rec = dict(rec)
+    line_rec["served"] = compact_served(rec["served"])
+    line_rec["extras"] = compact_extras(rec["extras"])
+    if cpu:
+        line_rec["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "blas_threads",
+                                                        "single_core_pts_per_s") if k in cpu}
+        line_rec["cpu_baseline"]["sample"] = (
+            "d/dx of all x-lines of the benchmarked 512^3 field, one LAPACK gtsv thread per process; "
+            "single core: 65536 lines per direction")
+    small = json.dumps(sig(line_rec), separators=(",", ":"))
     log(f"[bench] stdout line: {len(small)} bytes")
     print(small, flush=True)
 
