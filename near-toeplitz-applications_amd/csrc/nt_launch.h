@@ -6,6 +6,7 @@
 
 #include "nt_line_kernel.h"
 #include "nt_tile_kernel.h"
+#include "nt_xw_kernel.h"
 
 namespace nt {
 
@@ -22,6 +23,9 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //              stores, wave-private LDS transpose (needs 16-byte aligned lines)
 //           1: one line per wavefront, each lane loads its own chunk directly
 //           2: workgroup tile of 4 lines staged through LDS (any n, alignment)
+//           3: one line per workgroup over n/128 wavefronts, two rows per lane
+//              (power-of-two n from 256 to 2048, 16-byte aligned lines; other
+//              lines take variant 0's path)
 //   STRIDED 0: tile of 16 adjacent lines, 64 lanes per line (1024 threads)
 //           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads);
 //              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads);
@@ -37,7 +41,7 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 // launch-shape variants per tuning layout: 0 contiguous, 1 strided, 2 and 3 the
 // same for the ADI half steps (their right-hand side reads neighbouring lines,
 // which favours other shapes than the derivative's)
-constexpr int kNumVariants[4] = {3, 4, 3, 4};
+constexpr int kNumVariants[4] = {4, 4, 4, 4};
 int tune_get(int layout, int prec, int n);
 void tune_set(int layout, int prec, int n, int variant);
 
@@ -77,15 +81,45 @@ inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) ==
                (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
   } while (0)
 
+// Resident one-wavefront workgroups per CU of the wave-coalesced kernel,
+// capped through dynamic LDS (0: no cap).  Fewer lines in flight stream
+// faster: a copy with the kernel's access pattern (one 4 KB line per
+// wavefront) moves a 512^3 FP64 field at 6.2 TB/s with 32 workgroups per CU
+// and 6.7-6.8 TB/s with 6-8.  The kernel itself needs more to hide its solve;
+// measured on MI355X, 512^3 d/dx over four interleaved runs: no cap 343-349 us
+// (FP32 + refinement 333-361), 16: 333-335 (357-360), 22: 333.7-334.0
+// (335.6-337.4), 24: 332-335 (336-340), 8: 411 (514).
+#ifndef NT_WCT_WGS_PER_CU
+#define NT_WCT_WGS_PER_CU 22
+#endif
+inline size_t wct_lds_pad(size_t static_lds) {
+  constexpr int cap = NT_WCT_WGS_PER_CU;
+  if (cap <= 0) return 0;
+  const size_t per = (static_cast<size_t>(160 * 1024) / cap) & ~static_cast<size_t>(255);
+  return per > static_lds ? per - static_lds : 0;
+}
+
+#define NT_LAUNCH_LDS_(KERNEL, TS_, NT_, LDS_)                                             \
+  do {                                                                                     \
+    const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
+    a.tiles = (nsys_ + (TS_)-1) / (TS_);                                                   \
+    const int64_t grid_ = outer * a.tiles;                                                 \
+    if (grid_ <= 0) return hipSuccess;                                                     \
+    if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
+    hipLaunchKernelGGL(KERNEL, dim3(static_cast<unsigned>(grid_)), dim3(NT_), (LDS_), st, a); \
+    return hipGetLastError();                                                              \
+  } while (0)
+
 // wave-coalesced streaming kernel (CONTIG variant 0)
 #define NT_GOW(R_, W_, WAVES_)                                                             \
   do {                                                                                     \
     if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    const size_t lds_ = ((WAVES_) == 1) ? wct_lds_pad(static_cast<size_t>(W_) * ((R_) + 2) * 8) : 0; \
     if (is_full(n, W_, R_))                                                                \
-      NT_LAUNCH_((line_kernel_wct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, true>),           \
-                 (WAVES_) * (64 / (W_)), (WAVES_)*64);                                     \
-    NT_LAUNCH_((line_kernel_wct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, false>),            \
-               (WAVES_) * (64 / (W_)), (WAVES_)*64);                                       \
+      NT_LAUNCH_LDS_((line_kernel_wct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, true>),       \
+                     (WAVES_) * (64 / (W_)), (WAVES_)*64, lds_);                           \
+    NT_LAUNCH_LDS_((line_kernel_wct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, false>),        \
+                   (WAVES_) * (64 / (W_)), (WAVES_)*64, lds_);                             \
   } while (0)
 
 // strided-line tile kernel (STRIDED variants)
@@ -292,6 +326,20 @@ inline bool tile_f32_fast_ok(const LineArgs& a) {
     return hipGetLastError();                                                              \
   } while (0)
 
+// one line per workgroup over NW wavefronts (nt_xw_kernel.h); n == 128 * NW
+#define NT_GOX(NW_)                                                                        \
+  do {                                                                                     \
+    if (a.plan.nlev != 6 + Log2<NW_>::value) return hipErrorInvalidValue;                  \
+    const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
+    a.tiles = nsys_;                                                                       \
+    const int64_t grid_ = outer * nsys_;                                                   \
+    if (grid_ <= 0) return hipSuccess;                                                     \
+    if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
+    hipLaunchKernelGGL((line_kernel_xw<TIn, NW_, PROD, PREC>), dim3(static_cast<unsigned>(grid_)), \
+                       dim3((NW_)*64), 0, st, a);                                          \
+    return hipGetLastError();                                                              \
+  } while (0)
+
 // 16-byte chunk accesses need 16-byte aligned line starts
 inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
   const uintptr_t i = reinterpret_cast<uintptr_t>(a.in), o = reinterpret_cast<uintptr_t>(a.out);
@@ -306,7 +354,15 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
   if constexpr (LAYOUT == CONTIG) {
     const int vin = 16 / static_cast<int>(sizeof(TIn)), vout = 16 / static_cast<int>(sizeof(TOut));
     const bool aligned = n > 64 && direct_ok(a, sizeof(TIn), sizeof(TOut));
-    if (variant == 0 && aligned && a.pstride == n && n % vin == 0 && n % vout == 0) {
+    if constexpr (std::is_same_v<TOut, double>) {
+      if (variant == 3 && aligned && a.pstride == n && (n & (n - 1)) == 0 && n >= 256) {
+        if (n == 256) NT_GOX(2);
+        if (n == 512) NT_GOX(4);
+        if (n == 1024) NT_GOX(8);
+        if (n == 2048) NT_GOX(16);
+      }
+    }
+    if ((variant == 0 || variant == 3) && aligned && a.pstride == n && n % vin == 0 && n % vout == 0) {
       if (n <= 128) NT_GOW(2, 64, 4);
       if (n <= 256) NT_GOW(4, 64, 4);
       if (n <= 512) NT_GOW(8, 64, NT_WCT_WAVES_512);

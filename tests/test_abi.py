@@ -147,9 +147,9 @@ def test_tuning_table_set_get_and_validation():
     assert lib.nt_tune_set(1, 1, 700, keep.value) == 0
     # the ADI layouts (2, 3) have slots of their own
     adi_keep = ctypes.c_int()
-    assert lib.nt_tune_get(2, 0, 2048, ctypes.byref(adi_keep), ctypes.byref(nv)) == 0 and nv.value == 3
+    assert lib.nt_tune_get(2, 0, 2048, ctypes.byref(adi_keep), ctypes.byref(nv)) == 0 and nv.value == 4
     assert lib.nt_tune_get(0, 0, 2048, ctypes.byref(p2), None) == 0
-    assert lib.nt_tune_set(2, 0, 2048, (adi_keep.value + 1) % 3) == 0
+    assert lib.nt_tune_set(2, 0, 2048, (adi_keep.value + 1) % 4) == 0
     assert lib.nt_tune_get(0, 0, 2048, ctypes.byref(v), None) == 0 and v.value == p2.value
     assert lib.nt_tune_set(2, 0, 2048, adi_keep.value) == 0
     assert lib.nt_tune_get(3, 0, 64, None, ctypes.byref(nv)) == 0 and nv.value == 4
@@ -157,7 +157,7 @@ def test_tuning_table_set_get_and_validation():
     assert lib.nt_tune_set(-1, 0, 64, 0) == _lib.NT_EINVAL
     assert lib.nt_tune_set(0, 5, 64, 0) == _lib.NT_EINVAL
     assert lib.nt_tune_set(0, 0, 1, 0) == _lib.NT_EINVAL
-    assert lib.nt_tune_set(0, 0, 64, 3) == _lib.NT_EINVAL
+    assert lib.nt_tune_set(0, 0, 64, 4) == _lib.NT_EINVAL
     assert lib.nt_tune_set(1, 0, 64, 4) == _lib.NT_EINVAL
     assert b"variant" in lib.nt_last_error()
     assert lib.nt_tune_get(1, 0, 64, None, ctypes.byref(nv)) == 0 and nv.value == 4
