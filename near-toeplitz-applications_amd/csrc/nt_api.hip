@@ -49,9 +49,11 @@ namespace {
 // the first and 483 -> 538 us in the second (these shapes vary with where the
 // buffers land).
 int g_tune[4][2][13][2] = {
-    // contiguous lines: 0 wave-private transpose, 1 direct chunk loads, 2 workgroup tile
-    {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 1}, {1, 1}},
-     {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 1}, {0, 1}}},
+    // contiguous lines: 0 wave-private transpose, 1 direct chunk loads, 2 workgroup tile,
+    // 3 one line over several wavefronts, 4 persistent form of 0 (2048-point lines:
+    // FP64 442 -> 385 us, FP32 + refinement 536 -> 384 us on 2^27 points)
+    {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 4}, {1, 1}},
+     {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 4}, {0, 1}}},
     // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
     // 2 persistent whole-tile kernel, 3 the same with two lines per wavefront (two workgroups per CU)
     {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {2, 1}},
@@ -76,23 +78,33 @@ int log2_bucket(int n) {
 }
 }  // namespace
 
-int64_t nt::persistent_grid(const void* kernel, int threads) {
+int64_t nt::device_cus() {
   // compute units per device, cached (set once per device; a race only repeats the query)
   static std::atomic<int> cus_cache[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
   int cus = (dev < 64) ? cus_cache[dev].load(std::memory_order_relaxed) : 0;
   if (cus <= 0) {
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+      (void)hipGetLastError();
       cus = 256;
+    }
     if (dev < 64) cus_cache[dev].store(cus, std::memory_order_relaxed);
   }
+  return cus;
+}
+
+int nt::occupancy_per_cu(const void* kernel, int threads, size_t dyn_lds) {
   int per = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, dyn_lds) != hipSuccess || per < 1) {
     (void)hipGetLastError();
     per = 1;
   }
-  return static_cast<int64_t>(cus) * per;
+  return per;
+}
+
+int64_t nt::persistent_grid(const void* kernel, int threads) {
+  return device_cus() * occupancy_per_cu(kernel, threads, 0);
 }
 
 int nt::tune_get(int layout, int prec, int n) {

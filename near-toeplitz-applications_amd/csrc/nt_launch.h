@@ -26,6 +26,9 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //           3: one line per workgroup over n/128 wavefronts, two rows per lane
 //              (power-of-two n from 256 to 2048, 16-byte aligned lines; other
 //              lines take variant 0's path)
+//           4: persistent form of 0 with the next line prefetched, a few
+//              wavefronts per CU (power-of-two n from 128 (FP32 input: 256) to
+//              2048 outside ADI; other lines take variant 0's path)
 //   STRIDED 0: tile of 16 adjacent lines, 64 lanes per line (1024 threads)
 //           1: tile of 32 adjacent lines, 32 lanes per line (1024 threads);
 //              lines longer than 512: tile of 8 lines, 64 lanes per line (512 threads);
@@ -41,7 +44,7 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 // launch-shape variants per tuning layout: 0 contiguous, 1 strided, 2 and 3 the
 // same for the ADI half steps (their right-hand side reads neighbouring lines,
 // which favours other shapes than the derivative's)
-constexpr int kNumVariants[4] = {4, 4, 4, 4};
+constexpr int kNumVariants[4] = {5, 4, 4, 4};
 int tune_get(int layout, int prec, int n);
 void tune_set(int layout, int prec, int n, int variant);
 
@@ -340,6 +343,46 @@ inline bool tile_f32_fast_ok(const LineArgs& a) {
     return hipGetLastError();                                                              \
   } while (0)
 
+// Wavefronts per CU of the persistent wave-coalesced kernel (contiguous variant
+// 4), by rows per lane.  Measured on MI355X, compact d/dx of 2^27-point fields
+// (us at 4 / 8 / 16 wavefronts per CU; the default non-persistent kernel in
+// brackets), FP64: 128 points 1191 / 688 / 589 (555), 256 points 770 / 492 /
+// 431 (376), 512 points 474 / 395 / 399 (334), 1024 points 371 / 402 / 404
+// (385), 2048 points 385 / 395 / 394 (442); FP32 + refinement at 2048 points
+// 385 / 384 / 387 (536).  The persistent form is the default for 2048-point
+// lines only.
+#ifndef NT_WCP_PER_CU
+#define NT_WCP_PER_CU(R) (((R) <= 4) ? 16 : (((R) <= 8) ? 8 : 4))
+#endif
+int64_t device_cus();
+int occupancy_per_cu(const void* kernel, int threads, size_t dyn_lds);
+inline int wcp_per_cu(int r) {
+  const int c = NT_WCP_PER_CU(r);
+  return c < 1 ? 1 : c;
+}
+
+// The grid never exceeds the workgroups that fit at once (a persistent
+// workgroup that waits for a slot would start its whole share at the end).
+#define NT_GOWP(R_)                                                                        \
+  if constexpr ((R_) >= 16 / static_cast<int>(sizeof(TIn))) do {                          \
+    if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
+    const int64_t nsys_ = a.phys;                                                          \
+    a.tiles = nsys_;                                                                       \
+    if (nsys_ <= 0) return hipSuccess;                                                     \
+    const void* k_ = reinterpret_cast<const void*>(&line_kernel_wcp<TIn, R_, PROD, PREC>); \
+    const int want_ = wcp_per_cu(R_);                                                      \
+    const size_t slds_ = static_cast<size_t>(64) * ((R_) + 2) * 8;                         \
+    const size_t per_lds_ = (static_cast<size_t>(160 * 1024) / want_) & ~static_cast<size_t>(1023); \
+    const size_t lds_ = per_lds_ > slds_ ? per_lds_ - slds_ : 0;                           \
+    int per_ = occupancy_per_cu(k_, 64, lds_);                                             \
+    per_ = per_ < want_ ? per_ : want_;                                                    \
+    int64_t grid_ = device_cus() * per_;                                                   \
+    grid_ = grid_ < nsys_ ? grid_ : nsys_;                                                 \
+    hipLaunchKernelGGL((line_kernel_wcp<TIn, R_, PROD, PREC>), dim3(static_cast<unsigned>(grid_)), \
+                       dim3(64), lds_, st, a);                                             \
+    return hipGetLastError();                                                              \
+  } while (0)
+
 // 16-byte chunk accesses need 16-byte aligned line starts
 inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
   const uintptr_t i = reinterpret_cast<uintptr_t>(a.in), o = reinterpret_cast<uintptr_t>(a.out);
@@ -362,7 +405,17 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
         if (n == 2048) NT_GOX(16);
       }
     }
-    if ((variant == 0 || variant == 3) && aligned && a.pstride == n && n % vin == 0 && n % vout == 0) {
+    if constexpr (std::is_same_v<TOut, double> && PROD != P_ADI) {
+      if (variant == 4 && aligned && a.pstride == n && outer == 1 && (n & (n - 1)) == 0 &&
+          n >= 64 * vin) {
+        if (n == 128) NT_GOWP(2);
+        if (n == 256) NT_GOWP(4);
+        if (n == 512) NT_GOWP(8);
+        if (n == 1024) NT_GOWP(16);
+        if (n == 2048) NT_GOWP(32);
+      }
+    }
+    if ((variant == 0 || variant >= 3) && aligned && a.pstride == n && n % vin == 0 && n % vout == 0) {
       if (n <= 128) NT_GOW(2, 64, 4);
       if (n <= 256) NT_GOW(4, 64, 4);
       if (n <= 512) NT_GOW(8, 64, NT_WCT_WAVES_512);

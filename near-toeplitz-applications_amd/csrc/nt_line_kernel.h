@@ -618,6 +618,115 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
   }
 }
 
+// Persistent form of line_kernel_wct for whole power-of-two lines (n == 64 R,
+// one wavefront per workgroup, no halo lines).  Each wavefront walks lines
+// q, q + G, q + 2G, ... (G = the grid) and loads the next line into registers
+// while it solves and stores the current one.  The deposit of the next line
+// sits at the end of the loop body, after the stores, so its wait covers that
+// line's loads only and the stores keep draining; the last line is peeled off
+// the loop, so every load in the body is unconditional.  The grid is a few
+// wavefronts per CU: fewer lines in flight stream faster, and the prefetch
+// covers the solve that a low occupancy would otherwise expose.
+template <typename TIn, int R, int PROD, int PREC>
+__global__ void __launch_bounds__(64) line_kernel_wcp(const LineArgs args_by_value) {
+  (void)args_by_value;
+  static_assert(PROD == P_IDENT || PROD == P_COMPACT, "lines without halo lines");
+  const LineArgs& args = kernel_args();
+  constexpr int W = 64;
+  constexpr int N = W * R;
+  constexpr int CS = R + 2;                   // LDS doubles per chunk (2 pad: bank spread)
+  constexpr int VIN = 16 / static_cast<int>(sizeof(TIn));
+  constexpr int NV = N / VIN / 64;            // 16-byte loads per lane and line
+  static_assert(R >= VIN && N % (VIN * 64) == 0, "whole 16-byte loads inside chunks");
+  using VT = std::conditional_t<sizeof(TIn) == 8, nt_dbl2, nt_flt4>;
+  __shared__ __attribute__((aligned(16))) double L[W * CS];
+
+  const NtPlanK& P = args.plan;
+  const int lane = threadIdx.x;
+  const int64_t nsys = args.phys;
+  const int64_t G = gridDim.x;
+  int64_t q = blockIdx.x;                     // the launcher keeps the grid <= nsys
+  auto at = [](int e) -> int { return (e / R) * CS + (e % R); };
+
+  VT pf[NV];
+  auto fetch = [&](int64_t line) {
+    const VT* __restrict__ v = reinterpret_cast<const VT*>(static_cast<const TIn*>(args.in) + line * args.pstride);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) pf[k] = __builtin_nontemporal_load(v + k * 64 + lane);
+  };
+  auto deposit = [&]() {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int e = VIN * (k * 64 + lane);
+      if constexpr (VIN == 2) {
+        L[at(e)] = pf[k].x;
+        L[at(e + 1)] = pf[k].y;
+      } else {
+        L[at(e)] = static_cast<double>(pf[k].x);
+        L[at(e + 1)] = static_cast<double>(pf[k].y);
+        L[at(e + 2)] = static_cast<double>(pf[k].z);
+        L[at(e + 3)] = static_cast<double>(pf[k].w);
+      }
+    }
+  };
+  auto solve_store = [&](int64_t line) {
+    const int my_e0 = lane * R;
+    double* __restrict__ mine = L + lane * CS;
+    double f[R];
+#pragma unroll
+    for (int k = 0; k < R; k += 2) {
+      const nt_dbl2 t = *reinterpret_cast<const nt_dbl2*>(mine + k);
+      f[k] = t.x;
+      f[k + 1] = t.y;
+    }
+    double d[R];
+    if constexpr (PROD == P_COMPACT) {
+      double bnd[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) bnd[i] = L[at((i < 3) ? i : N + 2 - i)];
+      const double fl = shup<W>(f[R - 1], 1);
+      const double fr = shdn<W>(f[0], 1);
+      compact_rhs_chunk<R, true>(f, fl, fr, bnd, my_e0, N, args.s1, args.s2, d);
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; ++k) d[k] = f[k];
+    }
+    double x[R];
+    solve_chunk<R, W, PREC, true>(d, x, P, lane, args.sweeps);
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < R; k += 2) {
+      nt_dbl2 t;
+      t.x = x[k];
+      t.y = x[k + 1];
+      *reinterpret_cast<nt_dbl2*>(mine + k) = t;
+    }
+    wave_lds_sync();
+    nt_dbl2* __restrict__ dst = reinterpret_cast<nt_dbl2*>(static_cast<double*>(args.out) + line * args.pstride);
+#pragma unroll
+    for (int j = lane; j < N / 2; j += 64) {
+      nt_dbl2 t;
+      t.x = L[at(2 * j)];
+      t.y = L[at(2 * j + 1)];
+      __builtin_nontemporal_store(t, dst + j);
+    }
+  };
+
+  fetch(q);
+  deposit();
+  while (q + G < nsys) {                      // wave-uniform
+    const int64_t qn = q + G;
+    wave_lds_sync();
+    fetch(qn);
+    solve_store(q);
+    wave_lds_sync();
+    deposit();
+    q = qn;
+  }
+  wave_lds_sync();
+  solve_store(q);
+}
+
 // ---- strided lines (y/z): workgroup tile of TS adjacent lines ------------------------
 // The tile is TS lines that are adjacent in memory (unit stride between
 // lines, estride between elements).  Blocks of E elements of all staged
