@@ -112,6 +112,44 @@ def test_every_contiguous_variant_fp32_storage(dev, precision, n):
         assert np.abs(outs[v] - outs[0]).max() / scale < TOL[precision], v
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("n", [128, 256, 512, 1024, 2048])
+def test_many_lines_every_contiguous_variant(dev, precision, n, storage):
+    """More lines than the device holds wavefronts of the persistent kernel at
+    once (9001 lines: every wavefront of contiguous variant 4 walks several
+    lines, with the next one prefetched, and the count is ragged), every
+    contiguous variant against the oracle on 300 sampled lines (compact/rhs.py
+    + neato.py solve, FP64 or FP32 storage)."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.autotune import _set, current
+    from ntamd.workloads import compact_field
+    if storage == "f32" and n < 256:
+        pytest.skip("FP32 rows of the persistent kernel start at 256 points")
+    shape = (1, 9001, n)
+    f = compact_field(shape, seed=n + 3)
+    if storage == "f32":
+        f = f.astype(np.float32)
+    h = 1.0 / (n - 1)
+    rows = np.random.default_rng(n).choice(9001, 300, replace=False)
+    rows[:2] = (0, 9000)
+    want = compact.derivative_banded(f[:, rows, :].astype(np.float64), h, 2)
+    fd = torch.tensor(f, device=dev)
+    plan = Plan(n, COMPACT_COEFFS)
+    keep, nv = current(0, n, precision)
+    try:
+        for v in range(nv):
+            _set(0, precision, n, v)
+            out = torch.full(shape, float("nan"), dtype=torch.float64, device=dev)
+            core.compact_derivative(plan, fd, out, h, 2, precision)
+            got = out.cpu().numpy()
+            assert np.isfinite(got).all(), v
+            err = np.abs(got[:, rows, :] - want).max() / np.abs(want).max()
+            assert err < TOL[precision], (v, err)
+    finally:
+        _set(0, precision, n, keep)
+
+
 @pytest.mark.parametrize("N", [34, 130, 256, 300, 512, 700, 1024, 2048])
 @pytest.mark.parametrize("layout", [2, 3])
 def test_every_variant_adi(dev, N, layout):
