@@ -51,7 +51,12 @@ int nt_version(void);
  * ADI half steps, nt_adi_steps) at the given precision mode (NT_PREC_*).  Process-wide; the defaults are the shapes measured
  * fastest on MI355X, and the autotuner (ntamd/autotune.py) can re-measure and
  * set them from its JSON cache.  Every variant gives the same results to
- * within one rounding; only the launch shape changes. */
+ * within a few roundings; only the launch shape changes.  Contiguous layout:
+ * 0 one line per wavefront, 1 direct chunk loads, 2 workgroup tiles, 3 one
+ * line over several wavefronts, 4 persistent with the next line prefetched
+ * (not for ADI, whose layout 2 has variants 0-3).  Strided layouts: 0 16-line
+ * tiles, 1 32-line tiles, 2 persistent whole tiles, 3 the same with two lines
+ * per wavefront. */
 int nt_tune_set(int layout, int precision, int64_t n, int variant);
 int nt_tune_get(int layout, int precision, int64_t n, int* variant, int* nvariants);
 
