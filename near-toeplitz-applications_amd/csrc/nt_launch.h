@@ -92,13 +92,24 @@ inline bool is_full(int n, int W, int R) { return n == W * R && (n & (n - 1)) ==
 // measured on MI355X, 512^3 d/dx over four interleaved runs: no cap 343-349 us
 // (FP32 + refinement 333-361), 16: 333-335 (357-360), 22: 333.7-334.0
 // (335.6-337.4), 24: 332-335 (336-340), 8: 411 (514).
+//
+// 1024-point lines (four one-line wavefronts per workgroup) with the FP64 solve
+// run capped at 3 workgroups per CU: 2^27-point FP64 field 388 -> 373 us, the
+// 1024^3 FP32-stored field 2.394 -> 2.246 ms.  With FP32 + refinement the same
+// cap is slower (2.40 -> 2.71 ms), and 128- and 256-point lines gain nothing.
 #ifndef NT_WCT_WGS_PER_CU
 #define NT_WCT_WGS_PER_CU 22
 #endif
-inline size_t wct_lds_pad(size_t static_lds) {
-  constexpr int cap = NT_WCT_WGS_PER_CU;
+#ifndef NT_WCT_WGS_PER_CU_1024_F64
+#define NT_WCT_WGS_PER_CU_1024_F64 3
+#endif
+constexpr int wct_wgs_per_cu(int r, int waves, int prec) {
+  return (waves == 1) ? NT_WCT_WGS_PER_CU
+                      : ((r == 16 && waves == 4 && prec == PREC_F64) ? NT_WCT_WGS_PER_CU_1024_F64 : 0);
+}
+inline size_t wct_lds_pad(int cap, size_t static_lds) {
   if (cap <= 0) return 0;
-  const size_t per = (static_cast<size_t>(160 * 1024) / cap) & ~static_cast<size_t>(255);
+  const size_t per = (static_cast<size_t>(160 * 1024) / cap) & ~static_cast<size_t>(1023);
   return per > static_lds ? per - static_lds : 0;
 }
 
@@ -117,7 +128,8 @@ inline size_t wct_lds_pad(size_t static_lds) {
 #define NT_GOW(R_, W_, WAVES_)                                                             \
   do {                                                                                     \
     if ((R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                            \
-    const size_t lds_ = ((WAVES_) == 1) ? wct_lds_pad(static_cast<size_t>(W_) * ((R_) + 2) * 8) : 0; \
+    const size_t lds_ = wct_lds_pad(wct_wgs_per_cu(R_, WAVES_, PREC),                      \
+                                    static_cast<size_t>(WAVES_) * 64 * ((R_) + 2) * 8);    \
     if (is_full(n, W_, R_))                                                                \
       NT_LAUNCH_LDS_((line_kernel_wct<TIn, TOut, R_, W_, WAVES_, PROD, PREC, true>),       \
                      (WAVES_) * (64 / (W_)), (WAVES_)*64, lds_);                           \
