@@ -55,9 +55,13 @@ int g_tune[4][2][13][2] = {
     {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {1, 4}, {1, 1}},
      {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {1, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 4}, {0, 1}}},
     // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
-    // 2 persistent whole-tile kernel, 3 the same with two lines per wavefront (two workgroups per CU)
-    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {2, 1}},
-     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 3}, {3, 3}, {3, 2}, {2, 2}, {2, 1}, {1, 1}}},
+    // 2 persistent whole-tile kernel, 3 the same with two lines per wavefront (two workgroups per CU),
+    // 4 as 3 with the two lines solved at once on 32-lane segments (2^25-2^26-point fields,
+    // y and z: FP64 96 points 519-582 -> 331-336 us, 128: 163 -> 120-126, 160: 393-396 ->
+    // 286-298, 480: 243-244 -> 216-227; FP32 + refinement 96: 673-713 -> 478-480, 256: 267-285 ->
+    // 238-284, but 288-480 points slower: 2 x 8 rows per lane take 173 VGPRs)
+    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 4}, {4, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {2, 1}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {4, 1}, {4, 4}, {4, 4}, {3, 2}, {2, 2}, {2, 1}, {1, 1}}},
     // ADI x half step (contiguous lines), variants as layout 0: at 2048 points the
     // wave-private transpose, which forms the stencil from coalesced reads of the
     // neighbouring lines (ADI 2048^2 step 58.7 -> 52.3 us; the compact derivative

@@ -40,11 +40,14 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //              outside ADI; otherwise as 2.  ADI lines above 1024 points (2, 3):
 //              whole tiles of the solved lines only, right-hand side formed at
 //              load (measured no faster than the halo tiles at 1024 and below)
+//           4: as 3, the two lines of a wavefront solved at once on 32-lane
+//              segments (4..512-point lines outside ADI and FP32 storage;
+//              otherwise as 2)
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 // launch-shape variants per tuning layout: 0 contiguous, 1 strided, 2 and 3 the
 // same for the ADI half steps (their right-hand side reads neighbouring lines,
 // which favours other shapes than the derivative's)
-constexpr int kNumVariants[4] = {5, 4, 4, 4};
+constexpr int kNumVariants[4] = {5, 5, 4, 4};
 int tune_get(int layout, int prec, int n);
 void tune_set(int layout, int prec, int n, int variant);
 
@@ -236,6 +239,29 @@ int64_t persistent_grid(const void* kernel, int threads);
                          dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
     else                                                                                   \
       hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, LPW_, PROD, PREC, false>),     \
+                         dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
+    return hipGetLastError();                                                              \
+  } while (0)
+
+// Persistent whole-tile kernel, two lines per wavefront solved at once on
+// 32-lane segments (STRIDED variant 4).
+#define NT_GOTS(R_, WAVES_)                                                                \
+  do {                                                                                     \
+    if (2 * (R_) > (1 << a.plan.nlev)) return hipErrorInvalidValue;                        \
+    a.tiles = (a.phys + (WAVES_) * 2 - 1) / ((WAVES_) * 2);                                \
+    a.total = outer * a.tiles;                                                             \
+    if (a.total <= 0) return hipSuccess;                                                   \
+    const bool full_ = is_full(n, 64, R_);                                                 \
+    const void* k_ = full_                                                                 \
+        ? reinterpret_cast<const void*>(&line_kernel_tile<TIn, TOut, R_, WAVES_, 2, PROD, PREC, true, 2>)  \
+        : reinterpret_cast<const void*>(&line_kernel_tile<TIn, TOut, R_, WAVES_, 2, PROD, PREC, false, 2>); \
+    const int64_t cap_ = persistent_grid(k_, (WAVES_)*64);                                 \
+    const int64_t grid_ = a.total < cap_ ? a.total : cap_;                                 \
+    if (full_)                                                                             \
+      hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, 2, PROD, PREC, true, 2>),      \
+                         dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
+    else                                                                                   \
+      hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, 2, PROD, PREC, false, 2>),     \
                          dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
     return hipGetLastError();                                                              \
   } while (0)
@@ -458,6 +484,16 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
         if (n <= 128) NT_GOT(2, NT_TILE2_WAVES, 2);
         if (n <= 256) NT_GOT(4, NT_TILE2_WAVES, 2);
         if (n <= 512) NT_GOT(8, NT_TILE2_WAVES, 2);
+      }
+    }
+    if constexpr (PROD != P_ADI && !(std::is_same_v<TIn, float> && std::is_same_v<TOut, double> &&
+                                     PROD == P_COMPACT)) {
+      if (variant == 4) {
+        // as 3, both lines of a wavefront solved at once on 32-lane segments
+        if (n >= 4 && n <= 64) NT_GOTS(1, NT_TILE2_WAVES);
+        if (n > 64 && n <= 128) NT_GOTS(2, NT_TILE2_WAVES);
+        if (n > 128 && n <= 256) NT_GOTS(4, NT_TILE2_WAVES);
+        if (n > 256 && n <= 512) NT_GOTS(8, NT_TILE2_WAVES);
       }
     }
     if (variant >= 2) {

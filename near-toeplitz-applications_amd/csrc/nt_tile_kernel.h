@@ -84,7 +84,10 @@ __device__ inline TileWalk tile_walk(int64_t total, bool ranges) {
   return tw;
 }
 
-template <typename TIn, typename TOut, int R, int WAVES, int LPW, int PROD, int PREC, bool FULL>
+// SEG = 2 (with LPW = 2, no halo): a wavefront solves its two lines in one
+// pass, each on a 32-lane segment with 2 R rows per lane, instead of one after
+// the other on 64 lanes (fewer cross-lane levels per line).
+template <typename TIn, typename TOut, int R, int WAVES, int LPW, int PROD, int PREC, bool FULL, int SEG = 1>
 __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs args_by_value) {
   (void)args_by_value;
   const LineArgs& args = kernel_args();
@@ -93,6 +96,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
   constexpr int TS = WAVES * LPW;                // lines per tile
   constexpr int HALO = (PROD == P_ADI) ? 1 : 0;
   static_assert(LPW == 1 || HALO == 0, "ADI tiles hold one line per wavefront");
+  static_assert(SEG == 1 || (SEG == 2 && LPW == 2 && HALO == 0), "segments: two lines per wavefront, no halo");
   constexpr int ROWS = TS + 2 * HALO;            // staged lines
   constexpr int NMAX = W * R;
   // Chunk padding: lane l's chunk starts (R + CP) doubles after lane l-1's, an
@@ -219,6 +223,7 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
     if (EARLY && more) fetch(gn);                // the next tile's loads fly during this one
     __syncthreads();
 
+    if constexpr (SEG == 1) {
 #pragma unroll 1
     for (int h = 0; h < LPW; ++h) {
       const int ql = ql0 + h * WAVES;            // tile-local line of this wavefront
@@ -262,6 +267,41 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_tile(const LineArgs ar
 #pragma unroll
         for (int k = 0; k < R; ++k)
           if (FULL || my_e0 + k < n) lds[at(ql + HALO, my_e0 + k)] = x[k];
+      }
+    }
+    } else {
+      // both lines of the wavefront at once: lanes 0-31 line ql0, lanes 32-63
+      // line ql0 + WAVES, each with RS = 2 R rows per lane.  A segment whose line
+      // lies past the last one computes on zeros, so every value a shuffle can
+      // bring across the segment boundary is finite.
+      constexpr int WS = 32;
+      constexpr int RS = 2 * R;
+      const int sl = threadIdx.x & 31;           // lane inside the segment
+      const int ql = ql0 + ((threadIdx.x >> 5) & 1) * WAVES;
+      const bool active = (g.p0 + ql) < nsys;    // per segment
+      const int e0 = sl * RS;
+      double f[RS];
+#pragma unroll
+      for (int k = 0; k < RS; ++k) f[k] = (active && (FULL || e0 + k < n)) ? lds[at(ql, e0 + k)] : 0.0;
+      double d[RS];
+      if (PROD == P_COMPACT) {
+        double bnd[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bnd[i] = active ? lds[at(ql, (i < 3) ? i : n + 2 - i)] : 0.0;
+        const double fl = shup<WS>(f[RS - 1], 1);
+        const double fr = shdn<WS>(f[0], 1);
+        compact_rhs_chunk<RS, FULL>(f, fl, fr, bnd, e0, n, args.s1, args.s2, d);
+      } else {
+#pragma unroll
+        for (int k = 0; k < RS; ++k) d[k] = f[k];
+      }
+      double x[RS];
+      solve_chunk<RS, WS, PREC, FULL>(d, x, P, sl, args.sweeps);
+      wave_lds_sync();                           // the solution overwrites the lines just read
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < RS; ++k)
+          if (FULL || e0 + k < n) lds[at(ql, e0 + k)] = x[k];
       }
     }
     __syncthreads();
