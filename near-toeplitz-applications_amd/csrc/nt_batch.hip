@@ -187,20 +187,21 @@ __device__ __forceinline__ void cline_deposit(double* __restrict__ L, int n, int
 }
 
 // right-hand side (compact) or the values themselves (solve), then the solve
-template <int R, bool FULL, int PREC, int KIND>
+// on segments of W lanes
+template <int R, bool FULL, int PREC, int KIND, int W = 64>
 __device__ __forceinline__ void batch_solve(const double (&f)[R], const double (&bnd)[6], int my_e0, int n,
                                             double s1, double s2, const NtPlanK& P, int lane, int sweeps,
                                             double (&x)[R]) {
   double d[R];
   if constexpr (KIND != K_SOLVE) {
-    const double fl = shup<64>(f[R - 1], 1);
-    const double fr = shdn<64>(f[0], 1);
+    const double fl = shup<W>(f[R - 1], 1);
+    const double fr = shdn<W>(f[0], 1);
     compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, s1, s2, d);
   } else {
 #pragma unroll
     for (int k = 0; k < R; ++k) d[k] = f[k];
   }
-  solve_chunk<R, 64, PREC, FULL>(d, x, P, lane, sweeps);
+  solve_chunk<R, W, PREC, FULL>(d, x, P, lane, sweeps);
 }
 
 template <int R, bool FULL, int PREC, int KIND>
@@ -325,8 +326,13 @@ __device__ __forceinline__ void contig_chunk(CJob& J, const NtPlanK& P, int64_t 
 // ---- strided lines (y- and z-lines): 16-line whole tiles in LDS ------------------------
 // A tile is 16 adjacent lines, whose rows are runs of 16 consecutive elements
 // in memory; the whole tile sits in LDS and each of the 8 wavefronts solves
-// two of its lines.  The next tile's values are loaded into registers while
-// the current tile is solved and stored.
+// two of its lines, at once on two 32-lane segments with 2 R rows per lane
+// (up to 4 rows per lane; at 8 the doubled rows would spill in this kernel's
+// 128 VGPRs) or one after the other on 64 lanes.  The next tile's values are loaded into registers while the
+// current tile is solved and stored.
+#ifndef NT_BATCH_SEG_MAX_R
+#define NT_BATCH_SEG_MAX_R(PREC) 4
+#endif
 template <int R, bool FULL, int PREC, int KIND>
 __device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t t0, int64_t t1, double* lds) {
   using TIn = BatchIn<KIND>;
@@ -427,6 +433,32 @@ __device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t
     }
   };
   auto solve_tile = [&](const Tile& g) {
+    if constexpr (R <= NT_BATCH_SEG_MAX_R(PREC)) {
+      // lanes 0-31 line `wave`, lanes 32-63 line `wave + 8`; a segment past the
+      // tile's last line computes on zeros, so every value a shuffle brings
+      // across the segment boundary is finite
+      constexpr int RS = 2 * R;
+      const int sl = lane & 31;
+      const int ql = wave + (lane >> 5) * BATCH_WAVES;
+      const bool active = ql < g.rows;           // per segment
+      const int e0 = sl * RS;
+      double f[RS];
+#pragma unroll
+      for (int k = 0; k < RS; ++k) f[k] = (active && (FULL || e0 + k < n)) ? lds[at(ql, e0 + k)] : 0.0;
+      double bnd[6];
+      if constexpr (KIND != K_SOLVE) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bnd[i] = active ? lds[at(ql, (i < 3) ? i : n + 2 - i)] : 0.0;
+      }
+      double x[RS];
+      batch_solve<RS, FULL, PREC, KIND, 32>(f, bnd, e0, n, s1, s2, P, sl, sweeps, x);
+      wave_lds_sync();                           // the solution overwrites the lines just read
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < RS; ++k)
+          if (FULL || e0 + k < n) lds[at(ql, e0 + k)] = x[k];
+      }
+    } else {
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
       const int ql = wave + h * BATCH_WAVES;     // tile-local line of this wavefront
@@ -449,6 +481,7 @@ __device__ __forceinline__ void strided_chunk(CJob& J, const NtPlanK& P, int64_t
         for (int k = 0; k < R; ++k)
           if (FULL || my_e0 + k < n) lds[at(ql, my_e0 + k)] = x[k];
       }
+    }
     }
   };
   auto store = [&](const Tile& g) {
