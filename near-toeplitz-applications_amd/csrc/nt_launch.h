@@ -454,6 +454,15 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
       }
     }
     if ((variant == 0 || variant >= 3) && aligned && a.pstride == n && n % vin == 0 && n % vout == 0) {
+      if constexpr (PROD != P_ADI) {
+        // short lines on 16- or 32-lane segments, several lines per wavefront
+        // (fewer cross-lane levels per line): 2^26-point fields, FP64,
+        // 128 points 338 -> 178 us, 256 points 224 -> 181 us, 384 points
+        // 267 -> 229 us; 512 points keep 64 lanes (171 against 189 us)
+        if (n <= 128) NT_GOW(8, 16, 4);
+        if (n <= 256) NT_GOW(8, 32, 4);
+        if (n < 512) NT_GOW(16, 32, 1);
+      }
       if (n <= 128) NT_GOW(2, 64, 4);
       if (n <= 256) NT_GOW(4, 64, 4);
       if (n <= 512) NT_GOW(8, 64, NT_WCT_WAVES_512);
@@ -467,8 +476,15 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
       if (n <= 1024) NT_GOD(16, 64, 4);
       if (n <= 2048) NT_GOD(32, 64, 4);
     }
-    // LDS-staged workgroup tiles: one system per wavefront (16 lanes for tiny systems)
+    // LDS-staged workgroup tiles: one system per wavefront (16 lanes for tiny
+    // systems); short lines outside ADI on 16-lane segments, four lines per
+    // wavefront (2^26-point fields, FP64: 64 points 619 -> 233 us, 32 points
+    // 1059 -> 401 us)
     if (n <= 16) NT_GO(1, 16, 4, 16);
+    if constexpr (PROD != P_ADI) {
+      if (n <= 32) NT_GO(2, 16, 4, 32);
+      if (n <= 64) NT_GO(4, 16, 4, 64);
+    }
     if (n <= 64) NT_GO(1, 64, 4, 64);
     if (n <= 128) NT_GO(2, 64, 4, 128);
     if (n <= 256) NT_GO(4, 64, 4, 256);

@@ -536,6 +536,10 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
   wave_lds_sync();
 
   // ---- my chunk, right-hand side, solve ---------------------------------------------
+  // A segment past the wavefront's last line (SPW > 1, the grid's last
+  // wavefront) solves zeros: its LDS slice holds no line, and every value a
+  // shuffle brings across a segment boundary must be finite.
+  const bool seg_on = (SPW == 1) || seg < nl;
   const int my_e0 = lane * R;
   double f[R];
   double* __restrict__ mine = L + seg * LS + lane * CS;     // 16-byte aligned chunk
@@ -543,24 +547,24 @@ __global__ void __launch_bounds__(WAVES * 64) line_kernel_wct(const LineArgs arg
 #pragma unroll
     for (int k = 0; k < R; k += 2) {
       const nt_dbl2 t = *reinterpret_cast<const nt_dbl2*>(mine + k);
-      f[k] = t.x;
-      f[k + 1] = t.y;
+      f[k] = seg_on ? t.x : 0.0;
+      f[k + 1] = seg_on ? t.y : 0.0;
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < R; ++k) f[k] = (my_e0 + k < n) ? mine[k] : 0.0;
+    for (int k = 0; k < R; ++k) f[k] = (seg_on && my_e0 + k < n) ? mine[k] : 0.0;
   }
   double d[R];
   if (PROD == P_COMPACT) {
     double bnd[6];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) bnd[i] = L[at(seg, (i < 3) ? i : n + 2 - i)];
+    for (int i = 0; i < 6; ++i) bnd[i] = seg_on ? L[at(seg, (i < 3) ? i : n + 2 - i)] : 0.0;
     const double fl = shup<W>(f[R - 1], 1);
     const double fr = shdn<W>(f[0], 1);
     compact_rhs_chunk<R, FULL>(f, fl, fr, bnd, my_e0, n, args.s1, args.s2, d);
   } else if (PROD == P_ADI && sizeof(TIn) != 8) {
     const TIn* __restrict__ line =
-        static_cast<const TIn*>(args.in) + (q0 + seg + HALO) * args.pstride;
+        static_cast<const TIn*>(args.in) + (q0 + (seg_on ? seg : 0) + HALO) * args.pstride;
     double up[R], dn[R];
     load_chunk<TIn, R>(line - args.pstride, my_e0, n, up);
     load_chunk<TIn, R>(line + args.pstride, my_e0, n, dn);

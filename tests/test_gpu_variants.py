@@ -114,18 +114,18 @@ def test_every_contiguous_variant_fp32_storage(dev, precision, n):
 
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
-@pytest.mark.parametrize("n", [128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("n", [48, 100, 128, 256, 384, 512, 1024, 2048])
 def test_many_lines_every_contiguous_variant(dev, precision, n, storage):
     """More lines than the device holds wavefronts of the persistent kernel at
     once (9001 lines: every wavefront of contiguous variant 4 walks several
-    lines, with the next one prefetched, and the count is ragged), every
-    contiguous variant against the oracle on 300 sampled lines (compact/rhs.py
-    + neato.py solve, FP64 or FP32 storage)."""
+    lines, with the next one prefetched), and a line count that leaves the
+    last wavefront of the segmented shapes (2 or 4 lines per wavefront at 48,
+    100, 128, 256 and 384 points) partly empty: every contiguous variant
+    against the oracle on 300 sampled lines (compact/rhs.py + neato.py solve,
+    FP64 or FP32 storage)."""
     from ntamd import COMPACT_COEFFS, Plan, core
     from ntamd.autotune import _set, current
     from ntamd.workloads import compact_field
-    if storage == "f32" and n < 256:
-        pytest.skip("FP32 rows of the persistent kernel start at 256 points")
     shape = (1, 9001, n)
     f = compact_field(shape, seed=n + 3)
     if storage == "f32":
