@@ -323,6 +323,182 @@ __device__ __forceinline__ void contig_chunk(CJob& J, const NtPlanK& P, int64_t 
   }
 }
 
+// ---- contiguous lines of up to 256 points: several lines per wavefront -------------------
+// A wavefront takes SPW consecutive lines at once (4 lines on 16-lane segments
+// up to 128 points, 2 on 32-lane segments up to 256) and solves them together
+// with RS = R x SPW rows per lane: the same padded length as one line on 64
+// lanes, but fewer cross-lane levels per line.  Each line is read with the
+// whole wavefront's coalesced accesses into the wavefront's LDS rows (chunks
+// RS + 2 apart) and the next group of lines is fetched into registers during
+// the solve.  Lines past the chunk's end are fetched clamped to its last line
+// and solved as zeros (no shuffle then carries a non-finite value into a live
+// segment); their results are not stored.
+template <int R, bool FULL, int PREC, int KIND>
+__device__ __forceinline__ void contig_chunk_seg(CJob& J, const NtPlanK& P, int64_t t0, int64_t t1, double* lds) {
+  using TIn = BatchIn<KIND>;
+  constexpr int SPW = (R <= 2) ? 4 : 2;       // lines per wavefront
+  constexpr int W = 64 / SPW;                 // lanes per line
+  constexpr int RS = R * SPW;                 // rows per lane
+  constexpr int CS = RS + 2;                  // LDS doubles per chunk
+  constexpr int LSEG = W * CS;                // LDS doubles per line
+  static_assert(SPW * LSEG <= batch_ls(8), "the wavefront's LDS rows");
+  constexpr int V = 16 / static_cast<int>(sizeof(TIn));
+  constexpr int NV = (R >= V) ? R / V : 0;    // 16-byte loads per lane and line (0: scalar loads)
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int n = FULL ? 64 * R : P.n;
+  const int64_t phys = J.phys;
+  const int64_t pstride = J.pstride;
+  const int64_t q_end = (t1 * BATCH_TS < phys) ? t1 * BATCH_TS : phys;
+  int64_t q = t0 * BATCH_TS + static_cast<int64_t>(wave) * SPW;
+  if (q >= q_end) return;                     // uniform across the wavefront
+  const TIn* __restrict__ in = static_cast<const TIn*>(J.in);
+  double* __restrict__ out = static_cast<double*>(J.out);
+  const bool vec = (NV > 0) && J.vec != 0;
+  const double s1 = J.s1, s2 = J.s2;
+  const int sweeps = J.sweeps;
+  const int seg = lane / W;
+  const int sl = lane & (W - 1);
+  double* __restrict__ L = lds + wave * batch_ls(8);
+  auto at = [](int e) -> int { return (e / RS) * CS + (e % RS); };
+
+  TIn pf[SPW][R];
+  auto fetch = [&](int64_t q0, int ln) {
+#pragma unroll
+    for (int s = 0; s < SPW; ++s) {
+      const int64_t qs = (q0 + s < q_end) ? q0 + s : q_end - 1;
+      const TIn* __restrict__ src = in + qs * pstride;
+      if (vec) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          const int j = FULL ? ln + 64 * i : min(ln + 64 * i, n / V - 1);
+          if constexpr (sizeof(TIn) == 8) {
+            const nt_dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_dbl2*>(src) + j);
+            pf[s][2 * i] = v.x;
+            pf[s][2 * i + 1] = v.y;
+          } else {
+            const nt_flt4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_flt4*>(src) + j);
+            pf[s][4 * i] = v.x;
+            pf[s][4 * i + 1] = v.y;
+            pf[s][4 * i + 2] = v.z;
+            pf[s][4 * i + 3] = v.w;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const int e = ln + 64 * i;
+          pf[s][i] = __builtin_nontemporal_load(src + (FULL ? e : min(e, n - 1)));
+        }
+      }
+    }
+  };
+  auto deposit = [&](int ln) {
+#pragma unroll
+    for (int s = 0; s < SPW; ++s) {
+      double* __restrict__ Ls = L + s * LSEG;
+      if (vec) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          const int j = FULL ? ln + 64 * i : min(ln + 64 * i, n / V - 1);
+#pragma unroll
+          for (int c = 0; c < V; ++c) Ls[at(V * j + c)] = static_cast<double>(pf[s][V * i + c]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const int e = ln + 64 * i;
+          Ls[at(FULL ? e : min(e, n - 1))] = static_cast<double>(pf[s][i]);
+        }
+      }
+    }
+  };
+
+  fetch(q, (static_cast<int>(threadIdx.x) + opaque_vzero()) & 63);
+  for (;;) {
+    const int ln = (static_cast<int>(threadIdx.x) + opaque_vzero()) & 63;   // fresh per group
+    const int64_t qn = q + BATCH_WAVES * SPW;
+    const bool more = qn < q_end;               // uniform
+    wave_lds_sync();                            // the previous group's stores have read the rows
+    deposit(ln);
+    wave_lds_sync();
+    const bool on = q + seg < q_end;            // my segment holds a line
+    const double* __restrict__ mine = L + seg * LSEG + sl * CS;
+    const int e0 = sl * RS;
+    double f[RS];
+    if (FULL || e0 + RS <= n) {
+#pragma unroll
+      for (int k = 0; k < RS; k += 2) {
+        const nt_dbl2 t = *reinterpret_cast<const nt_dbl2*>(mine + k);
+        f[k] = on ? t.x : 0.0;
+        f[k + 1] = on ? t.y : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < RS; ++k) f[k] = (on && e0 + k < n) ? mine[k] : 0.0;
+    }
+    double bnd[6];
+    if constexpr (KIND != K_SOLVE) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) bnd[i] = on ? L[seg * LSEG + at((i < 3) ? i : n + 2 - i)] : 0.0;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) bnd[i] = 0.0;
+    }
+    if (more) fetch(qn, ln);                    // in flight during the solve
+    double x[RS];
+    batch_solve<RS, FULL, PREC, KIND, W>(f, bnd, e0, n, s1, s2, P, sl, sweeps, x);
+    wave_lds_sync();                            // every lane has read its chunk and the closures
+    double* __restrict__ mw = L + seg * LSEG + sl * CS;
+    if (FULL || e0 + RS <= n) {
+#pragma unroll
+      for (int k = 0; k < RS; k += 2) {
+        nt_dbl2 t;
+        t.x = x[k];
+        t.y = x[k + 1];
+        *reinterpret_cast<nt_dbl2*>(mw + k) = t;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < RS; ++k)
+        if (e0 + k < n) mw[k] = x[k];
+    }
+    wave_lds_sync();
+    const int ls = (static_cast<int>(threadIdx.x) + opaque_vzero()) & 63;   // fresh, as ln
+#pragma unroll
+    for (int s = 0; s < SPW; ++s) {
+      if (q + s < q_end) {                      // uniform
+        const double* __restrict__ Ls = L + s * LSEG;
+        double* __restrict__ dst = out + (q + s) * pstride;
+        if (J.vec != 0 && (n % 2) == 0) {
+          // 16-byte stores: 32 R of them cover the longest line, so at R = 1
+          // only half the lanes store even for a full line
+          constexpr int NVS = (32 * R + 63) / 64;
+          constexpr bool ALL = FULL && (32 * R) % 64 == 0;
+#pragma unroll
+          for (int i = 0; i < NVS; ++i) {
+            const int j = ls + 64 * i;
+            if (ALL || j < n / 2) {
+              nt_dbl2 t;
+              t.x = Ls[at(2 * j)];
+              t.y = Ls[at(2 * j + 1)];
+              __builtin_nontemporal_store(t, reinterpret_cast<nt_dbl2*>(dst) + j);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < R; ++i) {
+            const int e = ls + 64 * i;
+            if (FULL || e < n) __builtin_nontemporal_store(Ls[at(e)], dst + e);
+          }
+        }
+      }
+    }
+    if (!more) break;
+    q = qn;
+  }
+}
+
 // ---- strided lines (y- and z-lines): 16-line whole tiles in LDS ------------------------
 // A tile is 16 adjacent lines, whose rows are runs of 16 consecutive elements
 // in memory; the whole tile sits in LDS and each of the 8 wavefronts solves
@@ -542,7 +718,10 @@ __device__ __forceinline__ void run_class(CJob& J, const NtPlanK& P, int64_t t0,
   constexpr bool FULL = ((CLS / 6) % 2) != 0;
   constexpr int R = 1 << ((CLS / 12) % 4);
   constexpr int LAYOUT = CLS / 48;
-  if constexpr (LAYOUT == CONTIG) contig_chunk<R, FULL, PREC, KIND>(J, P, t0, t1, lds);
+  if constexpr (LAYOUT == CONTIG) {
+    if constexpr (R <= 4) contig_chunk_seg<R, FULL, PREC, KIND>(J, P, t0, t1, lds);
+    else contig_chunk<R, FULL, PREC, KIND>(J, P, t0, t1, lds);
+  }
   else strided_chunk<R, FULL, PREC, KIND>(J, P, t0, t1, lds);
 }
 
