@@ -191,7 +191,7 @@ extern "C" {
 
 const char* nt_last_error(void) { return g_err; }
 
-int nt_version(void) { return 103; }
+int nt_version(void) { return 104; }
 
 int nt_tune_set(int layout, int precision, int64_t n, int variant) {
   if (layout < 0 || layout > 3) return fail(NT_EINVAL, "nt_tune_set: layout must be 0..3");
