@@ -50,12 +50,13 @@ namespace {
 // buffers land).
 int g_tune[4][2][13][2] = {
     // contiguous lines: 0 wave-private transpose, 1 direct chunk loads, 2 workgroup tile,
-    // 3 one line over several wavefronts, 4 persistent form of 0 (2048-point lines:
-    // FP64 442 -> 385 us, FP32 + refinement 536 -> 384 us on 2^27 points)
+    // 3 one line over two wavefronts, 4 persistent form of 0 (2048-point lines: FP64
+    // 442 -> 385 us on 2^27 points; FP32 + refinement 536 -> 384 us, and 3 takes 366
+    // against 428 us for 4 in one run)
     // 65..255-point lines take 0 on 16- or 32-lane segments (2^26 points, FP64: 96 points
     // 191 us against 425 us for 1, 160 points 222 against 324)
     {{{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {1, 4}, {1, 1}},
-     {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 4}, {0, 1}}},
+     {{2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {2, 2}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 3}, {0, 1}}},
     // strided lines: 0 16-line tiles of 64 lanes per line, 1 32-line tiles (8 lines above 512),
     // 2 persistent whole-tile kernel, 3 the same with two lines per wavefront (two workgroups per CU),
     // 4 as 3 with the two lines solved at once on 32-lane segments (2^25-2^26-point fields,

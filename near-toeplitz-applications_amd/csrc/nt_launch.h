@@ -23,7 +23,7 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //              stores, wave-private LDS transpose (needs 16-byte aligned lines)
 //           1: one line per wavefront, each lane loads its own chunk directly
 //           2: workgroup tile of 4 lines staged through LDS (any n, alignment)
-//           3: one line per workgroup over n/128 wavefronts, two rows per lane
+//           3: one line per workgroup over two wavefronts, n/128 rows per lane
 //              (power-of-two n from 256 to 2048, 16-byte aligned lines; other
 //              lines take variant 0's path)
 //           4: persistent form of 0 with the next line prefetched, a few
@@ -367,16 +367,17 @@ inline bool tile_f32_fast_ok(const LineArgs& a) {
     return hipGetLastError();                                                              \
   } while (0)
 
-// one line per workgroup over NW wavefronts (nt_xw_kernel.h); n == 128 * NW
-#define NT_GOX(NW_)                                                                        \
+// one line per workgroup over NW wavefronts of R rows per lane (nt_xw_kernel.h);
+// n == 64 R NW
+#define NT_GOX(R_, NW_)                                                                    \
   do {                                                                                     \
-    if (a.plan.nlev != 6 + Log2<NW_>::value) return hipErrorInvalidValue;                  \
+    if (a.plan.nlev != Log2<R_>::value + 5 + Log2<NW_>::value) return hipErrorInvalidValue; \
     const int64_t nsys_ = a.phys - 2 * ((PROD == P_ADI) ? 1 : 0);                          \
     a.tiles = nsys_;                                                                       \
     const int64_t grid_ = outer * nsys_;                                                   \
     if (grid_ <= 0) return hipSuccess;                                                     \
     if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
-    hipLaunchKernelGGL((line_kernel_xw<TIn, NW_, PROD, PREC>), dim3(static_cast<unsigned>(grid_)), \
+    hipLaunchKernelGGL((line_kernel_xw<TIn, R_, NW_, PROD, PREC>), dim3(static_cast<unsigned>(grid_)), \
                        dim3((NW_)*64), 0, st, a);                                          \
     return hipGetLastError();                                                              \
   } while (0)
@@ -437,10 +438,10 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     const bool aligned = n > 64 && direct_ok(a, sizeof(TIn), sizeof(TOut));
     if constexpr (std::is_same_v<TOut, double>) {
       if (variant == 3 && aligned && a.pstride == n && (n & (n - 1)) == 0 && n >= 256) {
-        if (n == 256) NT_GOX(2);
-        if (n == 512) NT_GOX(4);
-        if (n == 1024) NT_GOX(8);
-        if (n == 2048) NT_GOX(16);
+        if (n == 256) NT_GOX(2, 2);
+        if (n == 512) NT_GOX(4, 2);
+        if (n == 1024) NT_GOX(8, 2);
+        if (n == 2048) NT_GOX(16, 2);
       }
     }
     if constexpr (std::is_same_v<TOut, double> && PROD != P_ADI) {
