@@ -62,8 +62,12 @@ int g_tune[4][2][13][2] = {
     // 4 as 3 with the two lines solved at once on 32-lane segments (2^25-2^26-point fields,
     // y and z: FP64 96 points 519-582 -> 331-336 us, 128: 163 -> 120-126, 160: 393-396 ->
     // 286-298, 480: 243-244 -> 216-227; FP32 + refinement 96: 673-713 -> 478-480, 256: 267-285 ->
-    // 238-284, but 288-480 points slower: 2 x 8 rows per lane take 173 VGPRs)
-    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 4}, {4, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {2, 1}},
+    // 238-284, but 288-480 points slower: 2 x 8 rows per lane take 173 VGPRs),
+    // 5 one lane per line, block-partitioned (FP64 128-, 256- and 512-point lines;
+    // 2^25-2^27-point fields: 512 points d/dz 475-557 -> 434-440 us, d/dy 425-430 ->
+    // 403-409; 256 points d/dz 234-238 -> 200-201, d/dy 229-232 -> 195-196; 128 points
+    // d/dz 250-256 -> 210, d/dy 222-234 -> 187-188)
+    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 4}, {4, 5}, {4, 5}, {4, 5}, {2, 2}, {2, 1}, {2, 1}},
      {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {4, 1}, {4, 4}, {4, 4}, {3, 2}, {2, 2}, {2, 1}, {1, 1}}},
     // ADI x half step (contiguous lines), variants as layout 0: at 2048 points the
     // wave-private transpose, which forms the stencil from coalesced reads of the
@@ -140,6 +144,7 @@ LineArgs make_args(const nt_plan* p) {
   LineArgs a;
   std::memset(&a, 0, sizeof(a));
   a.plan = p->k;
+  a.lane = p->lane.m ? &p->lane : nullptr;
   return a;
 }
 
@@ -192,7 +197,7 @@ extern "C" {
 
 const char* nt_last_error(void) { return g_err; }
 
-int nt_version(void) { return 104; }
+int nt_version(void) { return 105; }
 
 int nt_tune_set(int layout, int precision, int64_t n, int variant) {
   if (layout < 0 || layout > 3) return fail(NT_EINVAL, "nt_tune_set: layout must be 0..3");

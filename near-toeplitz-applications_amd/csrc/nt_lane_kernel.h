@@ -1,0 +1,144 @@
+// Strided lines with one lane per line: the block-partitioned lane solve.
+//
+// A workgroup of NT_LANE_P wavefronts owns 64 lines that are adjacent in
+// memory (y- or z-lines of a C-ordered field); wavefront w holds rows
+// [w M, (w+1) M) of all 64 lines, one line per lane, M rows in registers.
+// Each row access of a wavefront is one 512-byte run inside one plane, so
+// z-lines (rows a plane apart) touch one page per load instruction, where a
+// 16-line LDS tile touches eight.  Nothing is transposed.
+//
+// Per workgroup (the factors are the plan's NtLaneTab, read by scalar loads
+// from the kernel arguments):
+//   1. load M rows; compact: first and last row to LDS, barrier, then the
+//      right-hand side in registers (compact/rhs.py:21-36, with the one-sided
+//      closures at the line ends);
+//   2. Thomas sweep of the block -> y (RHS-independent pivots);
+//   3. the reduced system's right-hand side (-y of the block's first and last
+//      rows) to LDS, barrier; wavefront 0 solves the 2P - 2 tridiagonal rows
+//      of every lane's line in place, barrier;
+//   4. x = y + alpha uh + beta lh (alpha: the previous block's last value,
+//      beta: the next block's first value), stored.
+// The arithmetic is the host's lane_solve() in nt_plan.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nt_line_kernel.h"
+#include "nt_plan.h"
+
+namespace nt {
+
+struct LaneArgs {
+  const double* in;
+  double* out;
+  int64_t estride;   // distance between the rows of a line (elements)
+  int64_t ostride;   // distance between slabs
+  int64_t phys;      // lines per slab
+  int64_t groups;    // groups of 64 lines per slab
+  double s1, s2;     // compact producer: 3/(4h), 1/(2h)
+  NtLaneTab t;
+};
+
+// The arguments read in place from the kernel-argument segment, so that every
+// table read is a scalar load.
+__device__ __forceinline__ const LaneArgs& lane_args() {
+  return *(const LaneArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+}
+
+template <int M, int PROD>
+__global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes(LaneArgs args_in) {
+  (void)args_in;
+  static_assert(M >= 4 && M <= NT_LANE_MAX_M, "rows per block");
+  static_assert(PROD == P_IDENT || PROD == P_COMPACT, "producer");
+  constexpr int NW = NT_LANE_P, NR = NT_LANE_NR;
+  constexpr int RED_UNROLL = (M <= 16) ? 2 : NR;
+  const LaneArgs& A = lane_args();
+  __shared__ double edge[2][NW][64];
+  __shared__ double red[NR][64];
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  const int64_t slab = static_cast<int64_t>(blockIdx.x) / A.groups;
+  const int64_t g0 = (static_cast<int64_t>(blockIdx.x) - slab * A.groups) * 64;
+  const int64_t left = A.phys - g0;   // lines in this group: 1..64 used
+  const bool on = lane < left;
+  // lanes past the last line re-read the last line and store nothing
+  const uint32_t off = static_cast<uint32_t>(on ? lane : left - 1) * 8u;
+  const int64_t row0 = slab * A.ostride + g0 + static_cast<int64_t>(w) * M * A.estride;
+  const int64_t rs = A.estride * 8;
+  const char* src = reinterpret_cast<const char*>(A.in + row0);
+  double v[M];
+#pragma unroll
+  for (int r = 0; r < M; ++r)
+    v[r] = __builtin_nontemporal_load(reinterpret_cast<const double*>(src + r * rs + off));
+
+  if constexpr (PROD == P_COMPACT) {
+    edge[0][w][lane] = v[0];
+    edge[1][w][lane] = v[M - 1];
+    __syncthreads();
+    const double fl = (w > 0) ? edge[1][w > 0 ? w - 1 : 0][lane] : 0.0;
+    const double fr = (w < NW - 1) ? edge[0][w < NW - 1 ? w + 1 : 0][lane] : 0.0;
+    const double s1 = A.s1, s2 = A.s2;
+    const double b0 = s2 * (-5.0 * v[0] + 4.0 * v[1] + v[2]);
+    const double bn = -s2 * (-5.0 * v[M - 1] + 4.0 * v[M - 2] + v[M - 3]);
+    double prev = fl;
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const double cur = v[r];
+      const double nxt = (r + 1 < M) ? v[r + 1] : fr;
+      v[r] = s1 * (nxt - prev);
+      prev = cur;
+    }
+    if (w == 0) v[0] = b0;
+    if (w == NW - 1) v[M - 1] = bn;
+  }
+
+  // block sweep: the first block has its own pivots, the last its own last row
+  {
+    const double* ip = A.t.ip[w == 0 ? 0 : 1];
+    const double* cp = A.t.cp[w == 0 ? 0 : 1];
+    const double ai = A.t.ai;
+    v[0] *= ip[0];
+#pragma unroll
+    for (int r = 1; r < M - 1; ++r) v[r] = (v[r] - ai * v[r - 1]) * ip[r];
+    const double al = (w == NW - 1) ? A.t.an : ai;
+    const double il = (w == NW - 1) ? A.t.ip_last : ip[M - 1];
+    v[M - 1] = (v[M - 1] - al * v[M - 2]) * il;
+#pragma unroll
+    for (int r = M - 2; r >= 0; --r) v[r] -= cp[r] * v[r + 1];
+  }
+
+  // reduced system: row 2w-1 is this block's first row, row 2w its last
+  if (w >= 1) red[2 * w - 1][lane] = -v[0];
+  if (w <= NW - 2) red[2 * w][lane] = -v[M - 1];
+  __syncthreads();
+  if (w == 0) {
+    double g = red[0][lane] * A.t.rip[0];
+    red[0][lane] = g;
+#pragma unroll RED_UNROLL
+    for (int i = 1; i < NR; ++i) {
+      g = (red[i][lane] - A.t.rsub[i] * g) * A.t.rip[i];
+      red[i][lane] = g;
+    }
+#pragma unroll RED_UNROLL
+    for (int i = NR - 2; i >= 0; --i) {
+      g = red[i][lane] - A.t.rcp[i] * g;
+      red[i][lane] = g;
+    }
+  }
+  __syncthreads();
+  const double alpha = (w >= 1) ? red[w >= 1 ? 2 * w - 1 : 0][lane] : 0.0;
+  const double beta = (w <= NW - 2) ? red[w <= NW - 2 ? 2 * w : 0][lane] : 0.0;
+  const double* uh = A.t.uh[w == NW - 1 ? 1 : 0];
+  const double* lh = A.t.lh[w == 0 ? 0 : 1];
+  char* dst = reinterpret_cast<char*>(A.out + row0);
+  if (on) {
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const double x = v[r] + alpha * uh[r] + beta * lh[r];
+      __builtin_nontemporal_store(x, reinterpret_cast<double*>(dst + r * rs + off));
+    }
+  }
+}
+
+}  // namespace nt

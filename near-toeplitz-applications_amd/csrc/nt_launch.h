@@ -6,6 +6,7 @@
 
 #include "nt_line_kernel.h"
 #include "nt_tile_kernel.h"
+#include "nt_lane_kernel.h"
 #include "nt_xw_kernel.h"
 
 namespace nt {
@@ -43,11 +44,15 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //           4: as 3, the two lines of a wavefront solved at once on 32-lane
 //              segments (4..512-point lines outside ADI and FP32 storage;
 //              otherwise as 2)
+//           5: one lane per line, 64 lines per workgroup of 16 wavefronts,
+//              block-partitioned solve (FP64 128-, 256- and 512-point lines
+//              with adjacent lines adjacent in memory, outside ADI;
+//              otherwise as 2)
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 // launch-shape variants per tuning layout: 0 contiguous, 1 strided, 2 and 3 the
 // same for the ADI half steps (their right-hand side reads neighbouring lines,
 // which favours other shapes than the derivative's)
-constexpr int kNumVariants[4] = {5, 5, 4, 4};
+constexpr int kNumVariants[4] = {5, 6, 4, 4};
 int tune_get(int layout, int prec, int n);
 void tune_set(int layout, int prec, int n, int variant);
 
@@ -240,6 +245,28 @@ int64_t persistent_grid(const void* kernel, int threads);
     else                                                                                   \
       hipLaunchKernelGGL((line_kernel_tile<TIn, TOut, R_, WAVES_, LPW_, PROD, PREC, false>),     \
                          dim3(static_cast<unsigned>(grid_)), dim3((WAVES_)*64), 0, st, a); \
+    return hipGetLastError();                                                              \
+  } while (0)
+
+// One lane per line, block-partitioned (STRIDED variant 5, nt_lane_kernel.h):
+// a workgroup per 64 adjacent lines of a slab.
+#define NT_GOL(M_)                                                                         \
+  do {                                                                                     \
+    LaneArgs la_;                                                                          \
+    la_.in = static_cast<const double*>(a.in);                                             \
+    la_.out = static_cast<double*>(a.out);                                                 \
+    la_.estride = a.estride;                                                               \
+    la_.ostride = a.ostride;                                                               \
+    la_.phys = a.phys;                                                                     \
+    la_.groups = (a.phys + 63) / 64;                                                       \
+    la_.s1 = a.s1;                                                                         \
+    la_.s2 = a.s2;                                                                         \
+    la_.t = *a.lane;                                                                       \
+    const int64_t grid_ = outer * la_.groups;                                              \
+    if (grid_ <= 0) return hipSuccess;                                                     \
+    if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
+    hipLaunchKernelGGL((line_kernel_lanes<M_, PROD>), dim3(static_cast<unsigned>(grid_)),  \
+                       dim3(NT_LANE_P * 64), 0, st, la_);                                  \
     return hipGetLastError();                                                              \
   } while (0)
 
@@ -493,6 +520,14 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     if (n <= 1024) NT_GO(16, 64, 4, 1024);
     if (n <= 2048) NT_GO(32, 64, 4, 2048);
   } else {
+    if constexpr (std::is_same_v<TIn, double> && std::is_same_v<TOut, double> && PROD != P_ADI &&
+                  PREC == PREC_F64) {
+      if (variant == 5 && a.lane && a.lane->m * NT_LANE_P == n && a.pstride == 1) {
+        if (n == 8 * NT_LANE_P) NT_GOL(8);
+        if (n == 16 * NT_LANE_P) NT_GOL(16);
+        if (n == 32 * NT_LANE_P) NT_GOL(32);
+      }
+    }
     if constexpr (PROD != P_ADI && !(std::is_same_v<TIn, float> && std::is_same_v<TOut, double> &&
                                      PROD == P_COMPACT)) {
       if (variant == 3) {

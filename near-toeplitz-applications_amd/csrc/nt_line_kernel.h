@@ -42,6 +42,7 @@ struct LineArgs {
   double s1, s2;       // producer scalars: compact 3/(4h), 1/(2h); ADI r/2, 1-r
   int32_t sweeps;      // refinement sweeps (PREC_F32R)
   int32_t pad_;
+  const NtLaneTab* lane;   // host copy of the plan's lane-solve tables or null (never read on the device)
 };
 
 // The launch arguments read in place from the kernel-argument segment (the

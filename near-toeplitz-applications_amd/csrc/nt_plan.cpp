@@ -138,5 +138,131 @@ int nt_plan_build(nt_plan* p, int64_t n, const double coeffs[7], char* err, int 
   p->k.ci = coeffs[4]; p->k.an = coeffs[5]; p->k.bn = coeffs[6];
   fill_top(p->k.d.top, s);
   fill_top(p->k.f.top, s);
+  nt_lane_build(&p->lane, n, coeffs);
   return 0;
+}
+
+namespace {
+
+// One block of the lane solve, as the kernel runs it: forward
+// y_0 = d_0 ip_0, y_r = (d_r - a_r y_{r-1}) ip_r, then y_r -= cp_r y_{r+1}.
+// kind: 0 first block, 1 interior, 2 last.
+void lane_block(const NtLaneTab& t, int kind, const double* d, double* y) {
+  const int m = t.m;
+  const double* ip = t.ip[kind == 0 ? 0 : 1];
+  const double* cp = t.cp[kind == 0 ? 0 : 1];
+  y[0] = d[0] * ip[0];
+  for (int r = 1; r < m; ++r) {
+    const bool last = (kind == 2 && r == m - 1);
+    y[r] = (d[r] - (last ? t.an : t.ai) * y[r - 1]) * (last ? t.ip_last : ip[r]);
+  }
+  for (int r = m - 2; r >= 0; --r) y[r] -= cp[r] * y[r + 1];
+}
+
+// The whole lane solve of one line on the host (the kernel's arithmetic).
+void lane_solve(const NtLaneTab& t, const double* d, double* x) {
+  const int P = NT_LANE_P, m = t.m, NR = NT_LANE_NR;
+  double y[NT_LANE_P * NT_LANE_MAX_M], g[NT_LANE_NR];
+  for (int p = 0; p < P; ++p) lane_block(t, p == 0 ? 0 : (p == P - 1 ? 2 : 1), d + p * m, y + p * m);
+  for (int i = 0; i < NR; ++i) {
+    const int p = (i + 1) / 2;
+    const double r = (i % 2 == 1) ? -y[p * m] : -y[p * m + m - 1];
+    g[i] = (r - (i ? t.rsub[i] * g[i - 1] : 0.0)) * t.rip[i];
+  }
+  for (int i = NR - 2; i >= 0; --i) g[i] -= t.rcp[i] * g[i + 1];
+  for (int p = 0; p < P; ++p) {
+    const double al = (p >= 1) ? g[2 * p - 1] : 0.0, be = (p <= P - 2) ? g[2 * p] : 0.0;
+    const double* uh = t.uh[p == P - 1 ? 1 : 0];
+    const double* lh = t.lh[p == 0 ? 0 : 1];
+    for (int r = 0; r < m; ++r) x[p * m + r] = y[p * m + r] + al * uh[r] + be * lh[r];
+  }
+}
+
+bool lane_off(NtLaneTab* t) {
+  t->m = 0;
+  return false;
+}
+
+}  // namespace
+
+bool nt_lane_build(NtLaneTab* t, int64_t n, const double coeffs[7]) {
+  std::memset(t, 0, sizeof(*t));
+  const int P = NT_LANE_P, NR = NT_LANE_NR;
+  if (n % P != 0) return false;
+  const int m = static_cast<int>(n / P);
+  if (m != 8 && m != 16 && m != 32) return false;
+  const double b1 = coeffs[0], c1 = coeffs[1], ai = coeffs[2], bi = coeffs[3], ci = coeffs[4],
+               an = coeffs[5], bn = coeffs[6];
+  t->m = m;
+  t->ai = ai;
+  t->an = an;
+  // block sweeps: the first block starts with (b1, c1), the others with (bi, ci)
+  for (int k = 0; k < 2; ++k) {
+    double cprev = 0.0;
+    for (int r = 0; r < m; ++r) {
+      const bool first = (k == 0 && r == 0);
+      const double piv = (first ? b1 : bi) - (r ? ai * cprev : 0.0);
+      if (!finite_nonzero(piv)) return lane_off(t);
+      t->ip[k][r] = 1.0 / piv;
+      t->cp[k][r] = (first ? c1 : ci) / piv;
+      cprev = t->cp[k][r];
+    }
+  }
+  const double pl = bn - an * t->cp[1][m - 2];
+  if (!finite_nonzero(pl)) return lane_off(t);
+  t->ip_last = 1.0 / pl;
+  // responses to the couplings: uh = A^-1 (-ai e_0), lh = A^-1 (-ci e_{m-1})
+  double e[NT_LANE_MAX_M] = {};
+  e[0] = -ai;
+  lane_block(*t, 1, e, t->uh[0]);
+  lane_block(*t, 2, e, t->uh[1]);
+  e[0] = 0.0;
+  e[m - 1] = -ci;
+  lane_block(*t, 0, e, t->lh[0]);
+  lane_block(*t, 1, e, t->lh[1]);
+  // reduced system, row i for z_{i+1}: odd i: first row of block (i+1)/2,
+  // x_p[0] = z_{2p-1}; even i: last row of block i/2, x_p[m-1] = z_{2p+2}
+  double cprev = 0.0;
+  for (int i = 0; i < NR; ++i) {
+    double sub, dia, sup;
+    if (i % 2 == 1) {
+      const int p = (i + 1) / 2;
+      sub = -1.0;
+      dia = t->uh[p == P - 1 ? 1 : 0][0];
+      sup = (p <= P - 2) ? t->lh[1][0] : 0.0;
+    } else {
+      const int p = i / 2;
+      sub = (p >= 1) ? t->uh[0][m - 1] : 0.0;
+      dia = t->lh[p == 0 ? 0 : 1][m - 1];
+      sup = -1.0;
+    }
+    const double piv = dia - (i ? sub * cprev : 0.0);
+    if (!finite_nonzero(piv)) return lane_off(t);
+    t->rsub[i] = sub;
+    t->rip[i] = 1.0 / piv;
+    t->rcp[i] = sup / piv;
+    cprev = t->rcp[i];
+  }
+  // self-check against a whole-system Thomas solve of a test vector
+  double d[NT_LANE_P * NT_LANE_MAX_M], x[NT_LANE_P * NT_LANE_MAX_M], cw[NT_LANE_P * NT_LANE_MAX_M],
+      dw[NT_LANE_P * NT_LANE_MAX_M];
+  for (int i = 0; i < n; ++i) d[i] = std::sin(0.7 * i) + 0.3 * std::cos(2.1 * i + 0.4);
+  for (int i = 0; i < n; ++i) {
+    const double a = (i == n - 1) ? an : ai, b = (i == 0) ? b1 : (i == n - 1 ? bn : bi),
+                 c = (i == 0) ? c1 : ci;
+    const double piv = b - (i ? a * cw[i - 1] : 0.0);
+    if (!finite_nonzero(piv)) return lane_off(t);
+    cw[i] = c / piv;
+    dw[i] = (d[i] - (i ? a * dw[i - 1] : 0.0)) / piv;
+  }
+  for (int i = static_cast<int>(n) - 2; i >= 0; --i) dw[i] -= cw[i] * dw[i + 1];
+  lane_solve(*t, d, x);
+  double emax = 0.0, xmax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!std::isfinite(x[i]) || !std::isfinite(dw[i])) return lane_off(t);
+    emax = std::fmax(emax, std::fabs(x[i] - dw[i]));
+    xmax = std::fmax(xmax, std::fabs(dw[i]));
+  }
+  if (!(emax <= 1e-11 * xmax)) return lane_off(t);
+  return true;
 }

@@ -58,6 +58,34 @@ struct NtPlanK {
   NtPlanT<float> f;                    // the same factors rounded to FP32
 };
 
+// Block-partitioned solve with one lane per line (strided lines whose adjacent
+// lines are adjacent in memory): the n rows are cut into NT_LANE_P blocks of
+// m = n / NT_LANE_P rows, one wavefront per block, each lane a line.  Each block
+// is solved on its own by a Thomas sweep (y), then
+//   x = y + alpha * uh + beta * lh
+// with alpha the previous block's last value and beta the next block's first
+// (uh, lh: the block's responses to its two couplings, RHS-independent).  The
+// unknowns z = (alpha_0, beta_0, alpha_1, ...) satisfy the tridiagonal reduced
+// system of the reference's partition method (parallelcompact/reduced.py:163-178;
+// alpha_0 = beta_{P-1} = 0 are dropped, leaving 2P - 2 rows).  All factors are
+// RHS-independent and computed here once, in FP64.
+#define NT_LANE_P 16
+#define NT_LANE_MAX_M 32
+#define NT_LANE_NR (2 * NT_LANE_P - 2)
+struct NtLaneTab {
+  int32_t m;                        // rows per block; 0: no lane solve for this plan
+  int32_t pad_;
+  double ai, an;                    // sub-diagonal of block rows >= 1 (an: last row of the last block)
+  double ip[2][NT_LANE_MAX_M];      // 1 / pivot of the block sweep: [0] first block, [1] the others
+  double cp[2][NT_LANE_MAX_M];      // super-diagonal / pivot (the last block shares [1])
+  double ip_last;                   // 1 / pivot of the last block's last row
+  double uh[2][NT_LANE_MAX_M];      // response to alpha: [0] interior blocks, [1] last block
+  double lh[2][NT_LANE_MAX_M];      // response to beta: [0] first block, [1] interior blocks
+  double rsub[NT_LANE_NR];          // reduced system: sub-diagonal,
+  double rip[NT_LANE_NR];           //   1 / pivot,
+  double rcp[NT_LANE_NR];           //   super-diagonal / pivot
+};
+
 // Host-side plan object behind the opaque C handle.
 struct nt_plan {
   int64_t n;
@@ -65,8 +93,14 @@ struct nt_plan {
   int nlevels;                       // number of CR levels including level 0
   double lev[NT_MAX_LEVELS][8];      // m, bF, cF, a, b, c, aL, bL per level
   NtPlanK k;
+  NtLaneTab lane;                    // block-partitioned lane solve (lane.m == 0: none)
 };
 
 // Fill `p` for an n x n near-Toeplitz matrix.  Returns 0, or a negative code
 // with a message in `err` (zero pivot, n out of range, non-finite input).
 int nt_plan_build(nt_plan* p, int64_t n, const double coeffs[7], char* err, int errlen);
+
+// Fill t for n = NT_LANE_P * m (m in 8, 16, 32) and the matrix coeffs; returns
+// false (t->m = 0) for other n, a zero pivot, or when the block solve of a test
+// vector differs from a whole-system Thomas solve by more than 1e-11 relative.
+bool nt_lane_build(NtLaneTab* t, int64_t n, const double coeffs[7]);

@@ -9,7 +9,11 @@
 //   * the top system has 2 or 3 equations;
 //   * every factor is finite and the FP32 table is the FP64 one rounded;
 //   * diagonal dominance of the interior rows survives every reduction for
-//     the diagonally dominant families (cyclic reduction preserves it).
+//     the diagonally dominant families (cyclic reduction preserves it);
+//   * the lane-solve tables (n = 128, 256, 512): every block response solves
+//     its block equations, and the lane kernel's arithmetic restated here
+//     matches a band elimination with partial pivoting of the whole matrix;
+//     other n, and a zero block pivot, leave no lane tables.
 // Prints one summary line and exits 0 when every check holds.
 #include <cmath>
 #include <cstdio>
@@ -69,6 +73,116 @@ void check_plan(const nt_plan& p, long long n, bool dominant) {
     if (!std::isfinite(p.k.d.top.t[i])) fail("finite top factors", n);
 }
 
+// Tridiagonal solve by Gaussian elimination with partial pivoting (row i may
+// swap with row i+1; the swap fills one extra super-diagonal, kept in f).
+std::vector<double> gepp_solve(long long n, const double co[7], std::vector<double> x) {
+  std::vector<double> l(n, co[2]), d(n, co[3]), u(n, co[4]), f(n, 0.0);
+  d[0] = co[0];
+  u[0] = co[1];
+  l[n - 1] = co[5];   // l[i] couples row i+1 to column i: the last row's is an
+  d[n - 1] = co[6];
+  for (long long i = 0; i + 1 < n; ++i) {
+    const double li = (i + 1 == n - 1) ? co[5] : co[2];
+    if (std::fabs(d[i]) >= std::fabs(li)) {
+      const double t = li / d[i];
+      d[i + 1] -= t * u[i];
+      x[i + 1] -= t * x[i];
+    } else {
+      const double t = d[i] / li;
+      d[i] = li;
+      const double dn = d[i + 1];
+      d[i + 1] = u[i] - t * dn;
+      if (i + 2 < n) {
+        f[i] = u[i + 1];
+        u[i + 1] = -t * f[i];
+      }
+      u[i] = dn;
+      const double xi = x[i];
+      x[i] = x[i + 1];
+      x[i + 1] = xi - t * x[i + 1];
+    }
+  }
+  x[n - 1] /= d[n - 1];
+  if (n > 1) x[n - 2] = (x[n - 2] - u[n - 2] * x[n - 1]) / d[n - 2];
+  for (long long i = n - 3; i >= 0; --i) x[i] = (x[i] - u[i] * x[i + 1] - f[i] * x[i + 2]) / d[i];
+  return x;
+}
+
+// The lane kernel's arithmetic for one line: block sweeps, the reduced
+// system's forward and back sweeps, the correction.
+std::vector<double> lane_kernel_host(const NtLaneTab& t, const std::vector<double>& rhs) {
+  const int P = NT_LANE_P, m = t.m, NR = NT_LANE_NR;
+  std::vector<double> y(rhs), red(NR);
+  for (int w = 0; w < P; ++w) {
+    double* v = &y[w * m];
+    const double* ip = t.ip[w == 0 ? 0 : 1];
+    const double* cp = t.cp[w == 0 ? 0 : 1];
+    v[0] *= ip[0];
+    for (int r = 1; r < m - 1; ++r) v[r] = (v[r] - t.ai * v[r - 1]) * ip[r];
+    v[m - 1] = (v[m - 1] - (w == P - 1 ? t.an : t.ai) * v[m - 2]) * (w == P - 1 ? t.ip_last : ip[m - 1]);
+    for (int r = m - 2; r >= 0; --r) v[r] -= cp[r] * v[r + 1];
+    if (w >= 1) red[2 * w - 1] = -v[0];
+    if (w <= P - 2) red[2 * w] = -v[m - 1];
+  }
+  red[0] *= t.rip[0];
+  for (int i = 1; i < NR; ++i) red[i] = (red[i] - t.rsub[i] * red[i - 1]) * t.rip[i];
+  for (int i = NR - 2; i >= 0; --i) red[i] -= t.rcp[i] * red[i + 1];
+  for (int w = 0; w < P; ++w) {
+    const double al = (w >= 1) ? red[2 * w - 1] : 0.0, be = (w <= P - 2) ? red[2 * w] : 0.0;
+    for (int r = 0; r < m; ++r)
+      y[w * m + r] += al * t.uh[w == P - 1 ? 1 : 0][r] + be * t.lh[w == 0 ? 0 : 1][r];
+  }
+  return y;
+}
+
+// residual of one block's equations for the response v: first row (bf, cf),
+// interior (ai, bi, ci), last row (al, bl); rhs e0 on row 0, em on row m-1
+double block_residual(int m, double bf, double cf, double ai, double bi, double ci, double al, double bl,
+                      const double* v, double e0, double em) {
+  double worst = 0.0;
+  for (int r = 0; r < m; ++r) {
+    double ax;
+    if (r == 0) ax = bf * v[0] + cf * v[1];
+    else if (r == m - 1) ax = al * v[m - 2] + bl * v[m - 1];
+    else ax = ai * v[r - 1] + bi * v[r] + ci * v[r + 1];
+    const double want = (r == 0 ? e0 : 0.0) + (r == m - 1 ? em : 0.0);
+    worst = std::fmax(worst, std::fabs(ax - want));
+  }
+  return worst;
+}
+
+void check_lane(const nt_plan& p, long long n, const double co[7], bool expect) {
+  const NtLaneTab& t = p.lane;
+  const bool size_ok = (n == 128 || n == 256 || n == 512);
+  if (!(size_ok && expect)) {
+    if (t.m != 0) fail("no lane tables expected", n);
+    return;
+  }
+  if (t.m * NT_LANE_P != n) {
+    fail("lane tables built", n);
+    return;
+  }
+  const int m = t.m;
+  const double b1 = co[0], c1 = co[1], ai = co[2], bi = co[3], ci = co[4], an = co[5], bn = co[6];
+  const double scale = std::fabs(ai) + std::fabs(ci) + 1e-300;
+  // uh: the interior and last blocks' response to -ai on row 0; lh: the first
+  // and interior blocks' response to -ci on row m-1
+  if (block_residual(m, bi, ci, ai, bi, ci, ai, bi, t.uh[0], -ai, 0.0) > 1e-14 * scale ||
+      block_residual(m, bi, ci, ai, bi, ci, an, bn, t.uh[1], -ai, 0.0) > 1e-14 * scale ||
+      block_residual(m, b1, c1, ai, bi, ci, ai, bi, t.lh[0], 0.0, -ci) > 1e-14 * scale ||
+      block_residual(m, bi, ci, ai, bi, ci, ai, bi, t.lh[1], 0.0, -ci) > 1e-14 * scale)
+    fail("block responses solve their block equations", n);
+  std::vector<double> rhs(n);
+  for (long long i = 0; i < n; ++i) rhs[i] = std::cos(1.3 * i) - 0.5 * std::sin(0.37 * i * i);
+  const std::vector<double> want = gepp_solve(n, co, rhs), got = lane_kernel_host(t, rhs);
+  double e = 0.0, s = 0.0;
+  for (long long i = 0; i < n; ++i) {
+    e = std::fmax(e, std::fabs(got[i] - want[i]));
+    s = std::fmax(s, std::fabs(want[i]));
+  }
+  if (!(e <= 1e-12 * s)) fail("lane solve equals elimination with pivoting", n);
+}
+
 }  // namespace
 
 int main() {
@@ -92,11 +206,12 @@ int main() {
       fail(("compact: " + msg).c_str(), n);
     } else {
       check_plan(*plan, n, false);
+      check_lane(*plan, n, compact, true);
       ++built;
     }
     for (const auto& co : dominant_sets) {
       if (build(plan.get(), n, co.data(), 256, &msg) != 0) fail(("dominant: " + msg).c_str(), n);
-      else { check_plan(*plan, n, true); ++built; }
+      else { check_plan(*plan, n, true); check_lane(*plan, n, co.data(), true); ++built; }
     }
   }
   // sizes out of range
@@ -146,6 +261,13 @@ int main() {
   for (long long n : {2LL, 3LL, 5LL, 64LL, 100LL, 512LL, 2048LL}) {
     if (build(plan.get(), n, general, 64, nullptr) != 0) fail("general family", n);
     else check_plan(*plan, n, false);
+  }
+  // a zero pivot in the first block's sweep (b_i - a_i c1 / b1 = 0) leaves no lane tables
+  {
+    const double zero_block[7] = {1.0, 2.0, 0.5, 1.0, 0.25, 2.0, 1.0};
+    NtLaneTab t;
+    if (nt_lane_build(&t, 512, zero_block) || t.m != 0) fail("zero block pivot refused", 512);
+    if (!nt_lane_build(&t, 256, general) || t.m != 16) fail("general family lane tables", 256);
   }
   std::printf("plans built %lld, failures %d\n", built, failures);
   return failures == 0 ? 0 : 1;

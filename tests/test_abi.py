@@ -159,10 +159,10 @@ def test_tuning_table_set_get_and_validation():
     assert lib.nt_tune_set(0, 0, 1, 0) == _lib.NT_EINVAL
     assert lib.nt_tune_set(0, 0, 64, 5) == _lib.NT_EINVAL
     assert lib.nt_tune_set(2, 0, 64, 4) == _lib.NT_EINVAL
-    assert lib.nt_tune_set(1, 0, 64, 5) == _lib.NT_EINVAL
+    assert lib.nt_tune_set(1, 0, 64, 6) == _lib.NT_EINVAL
     assert lib.nt_tune_set(3, 0, 64, 4) == _lib.NT_EINVAL
     assert b"variant" in lib.nt_last_error()
-    assert lib.nt_tune_get(1, 0, 64, None, ctypes.byref(nv)) == 0 and nv.value == 5
+    assert lib.nt_tune_get(1, 0, 64, None, ctypes.byref(nv)) == 0 and nv.value == 6
 
 
 def test_noalloc_batch_limits_without_a_gpu():
