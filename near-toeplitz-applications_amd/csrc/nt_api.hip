@@ -63,11 +63,14 @@ int g_tune[4][2][13][2] = {
     // y and z: FP64 96 points 519-582 -> 331-336 us, 128: 163 -> 120-126, 160: 393-396 ->
     // 286-298, 480: 243-244 -> 216-227; FP32 + refinement 96: 673-713 -> 478-480, 256: 267-285 ->
     // 238-284, but 288-480 points slower: 2 x 8 rows per lane take 173 VGPRs),
-    // 5 one lane per line, block-partitioned (FP64 128-, 256- and 512-point lines;
-    // 2^25-2^27-point fields: 512 points d/dz 475-557 -> 434-440 us, d/dy 425-430 ->
-    // 403-409; 256 points d/dz 234-238 -> 200-201, d/dy 229-232 -> 195-196; 128 points
-    // d/dz 250-256 -> 210, d/dy 222-234 -> 187-188)
-    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 4}, {4, 5}, {4, 5}, {4, 5}, {2, 2}, {2, 1}, {2, 1}},
+    // 5 one lane per line, block-partitioned (FP64 solve of 49..512-point lines; shorter
+    // ones take 1).  2^26-point fields, d/dz and d/dy against the previous default:
+    // 50 points 543 -> 305, 537 -> 258 us; 64: 327 -> 217, 329 -> 214; 96: 336 -> 236,
+    // 331 -> 199; 160: 301 -> 211, 288 -> 190; 300: 265 -> 204, 259 -> 188; 384: 241 ->
+    // 182, 213 -> 181; 480: 192 -> 168, 179 -> 158; 512^3: 475-557 -> 427-443, 419-432 ->
+    // 395-412.  FP32-stored fields (FP64 solve, 2^27 points): 128 points 866 -> 328,
+    // 827 -> 286; 384: 581 -> 331, 541 -> 294; 512^3: 372 -> 321, 340 -> 312
+    {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {5, 5}, {5, 5}, {5, 5}, {5, 5}, {2, 2}, {2, 1}, {2, 1}},
      {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {4, 1}, {4, 4}, {4, 4}, {3, 2}, {2, 2}, {2, 1}, {1, 1}}},
     // ADI x half step (contiguous lines), variants as layout 0: at 2048 points the
     // wave-private transpose, which forms the stencil from coalesced reads of the

@@ -1,8 +1,10 @@
 // Strided lines with one lane per line: the block-partitioned lane solve.
 //
 // A workgroup of NT_LANE_P wavefronts owns 64 lines that are adjacent in
-// memory (y- or z-lines of a C-ordered field); wavefront w holds rows
-// [w M, (w+1) M) of all 64 lines, one line per lane, M rows in registers.
+// memory (y- or z-lines of a C-ordered field); wavefront w holds one block of
+// rows of all 64 lines, one line per lane, in registers: M rows for w < j and
+// M - 1 for the others (the plan's m and j; M = ceil(n / 16)), where the last
+// register row is padding that is never loaded or stored and holds zero.
 // Each row access of a wavefront is one 512-byte run inside one plane, so
 // z-lines (rows a plane apart) touch one page per load instruction, where a
 // 16-line LDS tile touches eight.  Nothing is transposed.
@@ -30,7 +32,7 @@
 namespace nt {
 
 struct LaneArgs {
-  const double* in;
+  const void* in;    // TIn: FP64 or FP32 values
   double* out;
   int64_t estride;   // distance between the rows of a line (elements)
   int64_t ostride;   // distance between slabs
@@ -46,51 +48,77 @@ __device__ __forceinline__ const LaneArgs& lane_args() {
   return *(const LaneArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
 }
 
-template <int M, int PROD>
+// EVEN: every block has M rows (n = 16 M), so no block carries a padding row.
+// TIn: the stored values (double, or float widened at load); the solve is FP64.
+template <typename TIn, int M, int PROD, bool EVEN>
 __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes(LaneArgs args_in) {
   (void)args_in;
   static_assert(M >= 4 && M <= NT_LANE_MAX_M, "rows per block");
   static_assert(PROD == P_IDENT || PROD == P_COMPACT, "producer");
   constexpr int NW = NT_LANE_P, NR = NT_LANE_NR;
-  constexpr int RED_UNROLL = (M <= 16) ? 2 : NR;
+  // The reduced system's loops unrolled by two keep its LDS reads from being
+  // issued all at once, which holds the kernel to 64 VGPRs (two workgroups per
+  // CU) up to 16 rows per lane (14 with padding rows): 52 instead of 98 VGPRs at
+  // 16.  Above that one workgroup fills a CU either way, and the fully unrolled
+  // loops hide the LDS latency of the serial reduced solve.
+  constexpr int RED_UNROLL = (M <= (EVEN ? 16 : 14)) ? 2 : NR;
   const LaneArgs& A = lane_args();
   __shared__ double edge[2][NW][64];
   __shared__ double red[NR][64];
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
+  const int jl = A.t.j;
+  const bool full = EVEN || w < jl;           // this block has M rows (else M - 1)
+  const bool lastw = (w == NW - 1);
   const int64_t slab = static_cast<int64_t>(blockIdx.x) / A.groups;
   const int64_t g0 = (static_cast<int64_t>(blockIdx.x) - slab * A.groups) * 64;
   const int64_t left = A.phys - g0;   // lines in this group: 1..64 used
   const bool on = lane < left;
   // lanes past the last line re-read the last line and store nothing
   const uint32_t off = static_cast<uint32_t>(on ? lane : left - 1) * 8u;
-  const int64_t row0 = slab * A.ostride + g0 + static_cast<int64_t>(w) * M * A.estride;
+  const int first_row = w * (M - 1) + (w < jl ? w : jl);
+  const int64_t row0 = slab * A.ostride + g0 + static_cast<int64_t>(first_row) * A.estride;
   const int64_t rs = A.estride * 8;
-  const char* src = reinterpret_cast<const char*>(A.in + row0);
+  const int64_t rsi = A.estride * static_cast<int64_t>(sizeof(TIn));
+  const uint32_t offi = off / (8 / sizeof(TIn));
+  const char* src = reinterpret_cast<const char*>(static_cast<const TIn*>(A.in) + row0);
   double v[M];
 #pragma unroll
-  for (int r = 0; r < M; ++r)
-    v[r] = __builtin_nontemporal_load(reinterpret_cast<const double*>(src + r * rs + off));
+  for (int r = 0; r < M - 1; ++r)
+    v[r] = static_cast<double>(__builtin_nontemporal_load(reinterpret_cast<const TIn*>(src + r * rsi + offi)));
+  if (full)
+    v[M - 1] = static_cast<double>(
+        __builtin_nontemporal_load(reinterpret_cast<const TIn*>(src + (M - 1) * rsi + offi)));
+  else
+    v[M - 1] = 0.0;
 
   if constexpr (PROD == P_COMPACT) {
     edge[0][w][lane] = v[0];
-    edge[1][w][lane] = v[M - 1];
+    edge[1][w][lane] = full ? v[M - 1] : v[M - 2];
     __syncthreads();
     const double fl = (w > 0) ? edge[1][w > 0 ? w - 1 : 0][lane] : 0.0;
     const double fr = (w < NW - 1) ? edge[0][w < NW - 1 ? w + 1 : 0][lane] : 0.0;
     const double s1 = A.s1, s2 = A.s2;
     const double b0 = s2 * (-5.0 * v[0] + 4.0 * v[1] + v[2]);
-    const double bn = -s2 * (-5.0 * v[M - 1] + 4.0 * v[M - 2] + v[M - 3]);
+    const double bn = full ? -s2 * (-5.0 * v[M - 1] + 4.0 * v[M - 2] + v[M - 3])
+                           : -s2 * (-5.0 * v[M - 2] + 4.0 * v[M - 3] + v[M - 4]);
     double prev = fl;
 #pragma unroll
     for (int r = 0; r < M; ++r) {
       const double cur = v[r];
-      const double nxt = (r + 1 < M) ? v[r + 1] : fr;
+      double nxt;
+      if (r + 2 < M) nxt = v[r + 1];
+      else if (r + 2 == M) nxt = full ? v[M - 1] : fr;
+      else nxt = fr;
       v[r] = s1 * (nxt - prev);
       prev = cur;
     }
+    if (!full) v[M - 1] = 0.0;
     if (w == 0) v[0] = b0;
-    if (w == NW - 1) v[M - 1] = bn;
+    if (lastw) {
+      if (full) v[M - 1] = bn;
+      else v[M - 2] = bn;
+    }
   }
 
   // block sweep: the first block has its own pivots, the last its own last row
@@ -100,17 +128,25 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes(LaneArgs arg
     const double ai = A.t.ai;
     v[0] *= ip[0];
 #pragma unroll
-    for (int r = 1; r < M - 1; ++r) v[r] = (v[r] - ai * v[r - 1]) * ip[r];
-    const double al = (w == NW - 1) ? A.t.an : ai;
-    const double il = (w == NW - 1) ? A.t.ip_last : ip[M - 1];
-    v[M - 1] = (v[M - 1] - al * v[M - 2]) * il;
+    for (int r = 1; r < M - 2; ++r) v[r] = (v[r] - ai * v[r - 1]) * ip[r];
+    {
+      const bool endrow = lastw && !full;     // row M-2 is the matrix's last row
+      const double a2 = endrow ? A.t.an : ai;
+      const double i2 = endrow ? A.t.ip_last : ip[M - 2];
+      v[M - 2] = (v[M - 2] - a2 * v[M - 3]) * i2;
+    }
+    if (full) {
+      const double a1 = lastw ? A.t.an : ai;
+      const double i1 = lastw ? A.t.ip_last : ip[M - 1];
+      v[M - 1] = (v[M - 1] - a1 * v[M - 2]) * i1;
+    }
 #pragma unroll
     for (int r = M - 2; r >= 0; --r) v[r] -= cp[r] * v[r + 1];
   }
 
   // reduced system: row 2w-1 is this block's first row, row 2w its last
   if (w >= 1) red[2 * w - 1][lane] = -v[0];
-  if (w <= NW - 2) red[2 * w][lane] = -v[M - 1];
+  if (w <= NW - 2) red[2 * w][lane] = full ? -v[M - 1] : -v[M - 2];
   __syncthreads();
   if (w == 0) {
     double g = red[0][lane] * A.t.rip[0];
@@ -129,14 +165,18 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes(LaneArgs arg
   __syncthreads();
   const double alpha = (w >= 1) ? red[w >= 1 ? 2 * w - 1 : 0][lane] : 0.0;
   const double beta = (w <= NW - 2) ? red[w <= NW - 2 ? 2 * w : 0][lane] : 0.0;
-  const double* uh = A.t.uh[w == NW - 1 ? 1 : 0];
-  const double* lh = A.t.lh[w == 0 ? 0 : 1];
+  const double* uh = A.t.uh[lastw ? 2 : (full ? 0 : 1)];
+  const double* lh = A.t.lh[w == 0 ? 0 : (full ? 1 : 2)];
   char* dst = reinterpret_cast<char*>(A.out + row0);
   if (on) {
 #pragma unroll
-    for (int r = 0; r < M; ++r) {
+    for (int r = 0; r < M - 1; ++r) {
       const double x = v[r] + alpha * uh[r] + beta * lh[r];
       __builtin_nontemporal_store(x, reinterpret_cast<double*>(dst + r * rs + off));
+    }
+    if (full) {
+      const double x = v[M - 1] + alpha * uh[M - 1] + beta * lh[M - 1];
+      __builtin_nontemporal_store(x, reinterpret_cast<double*>(dst + (M - 1) * rs + off));
     }
   }
 }

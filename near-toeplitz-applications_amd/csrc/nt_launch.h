@@ -45,9 +45,9 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //              segments (4..512-point lines outside ADI and FP32 storage;
 //              otherwise as 2)
 //           5: one lane per line, 64 lines per workgroup of 16 wavefronts,
-//              block-partitioned solve (FP64 128-, 256- and 512-point lines
-//              with adjacent lines adjacent in memory, outside ADI;
-//              otherwise as 2)
+//              block-partitioned FP64 solve (49- to 512-point lines with
+//              adjacent lines adjacent in memory, FP64 or FP32 storage, outside
+//              ADI; otherwise as 1 up to 48 points and as 4 above)
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 // launch-shape variants per tuning layout: 0 contiguous, 1 strided, 2 and 3 the
 // same for the ADI half steps (their right-hand side reads neighbouring lines,
@@ -249,11 +249,28 @@ int64_t persistent_grid(const void* kernel, int threads);
   } while (0)
 
 // One lane per line, block-partitioned (STRIDED variant 5, nt_lane_kernel.h):
-// a workgroup per 64 adjacent lines of a slab.
-#define NT_GOL(M_)                                                                         \
+// a workgroup per 64 adjacent lines of a slab, the instance for the plan's
+// block length m (4..32 rows).
+template <typename TIn, int PROD, int M = 4>
+hipError_t launch_lanes(const LaneArgs& la, int m, unsigned grid, hipStream_t st) {
+  if constexpr (M > NT_LANE_MAX_M) {
+    return hipErrorInvalidValue;
+  } else {
+    if (m == M) {
+      if (la.t.j == NT_LANE_P)
+        hipLaunchKernelGGL((line_kernel_lanes<TIn, M, PROD, true>), dim3(grid), dim3(NT_LANE_P * 64), 0, st, la);
+      else
+        hipLaunchKernelGGL((line_kernel_lanes<TIn, M, PROD, false>), dim3(grid), dim3(NT_LANE_P * 64), 0, st, la);
+      return hipGetLastError();
+    }
+    return launch_lanes<TIn, PROD, M + 1>(la, m, grid, st);
+  }
+}
+
+#define NT_GOL()                                                                           \
   do {                                                                                     \
     LaneArgs la_;                                                                          \
-    la_.in = static_cast<const double*>(a.in);                                             \
+    la_.in = a.in;                                                                         \
     la_.out = static_cast<double*>(a.out);                                                 \
     la_.estride = a.estride;                                                               \
     la_.ostride = a.ostride;                                                               \
@@ -265,9 +282,7 @@ int64_t persistent_grid(const void* kernel, int threads);
     const int64_t grid_ = outer * la_.groups;                                              \
     if (grid_ <= 0) return hipSuccess;                                                     \
     if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
-    hipLaunchKernelGGL((line_kernel_lanes<M_, PROD>), dim3(static_cast<unsigned>(grid_)),  \
-                       dim3(NT_LANE_P * 64), 0, st, la_);                                  \
-    return hipGetLastError();                                                              \
+    return launch_lanes<TIn, PROD>(la_, a.lane->m, static_cast<unsigned>(grid_), st);      \
   } while (0)
 
 // Persistent whole-tile kernel, two lines per wavefront solved at once on
@@ -459,7 +474,7 @@ inline bool direct_ok(const LineArgs& a, size_t in_size, size_t out_size) {
 template <typename TIn, typename TOut, int LAYOUT, int PROD, int PREC>
 hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
   const int n = a.plan.n;
-  const int variant = tune_get(LAYOUT + (PROD == P_ADI ? 2 : 0), PREC, n);
+  int variant = tune_get(LAYOUT + (PROD == P_ADI ? 2 : 0), PREC, n);
   if constexpr (LAYOUT == CONTIG) {
     const int vin = 16 / static_cast<int>(sizeof(TIn)), vout = 16 / static_cast<int>(sizeof(TOut));
     const bool aligned = n > 64 && direct_ok(a, sizeof(TIn), sizeof(TOut));
@@ -520,14 +535,13 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     if (n <= 1024) NT_GO(16, 64, 4, 1024);
     if (n <= 2048) NT_GO(32, 64, 4, 2048);
   } else {
-    if constexpr (std::is_same_v<TIn, double> && std::is_same_v<TOut, double> && PROD != P_ADI &&
-                  PREC == PREC_F64) {
-      if (variant == 5 && a.lane && a.lane->m * NT_LANE_P == n && a.pstride == 1) {
-        if (n == 8 * NT_LANE_P) NT_GOL(8);
-        if (n == 16 * NT_LANE_P) NT_GOL(16);
-        if (n == 32 * NT_LANE_P) NT_GOL(32);
-      }
+    if constexpr (std::is_same_v<TOut, double> && PROD != P_ADI && PREC == PREC_F64) {
+      if (variant == 5 && a.lane && a.lane->m >= 4 && a.pstride == 1) NT_GOL();
     }
+    // lines the lane kernel does not take: 32-line tiles below 49 points (33..48
+    // points, 2^26-point fields: 441-613 us against 523-744 for the segments),
+    // otherwise the segments
+    if (variant == 5) variant = (n <= 3 * NT_LANE_P) ? 1 : 4;
     if constexpr (PROD != P_ADI && !(std::is_same_v<TIn, float> && std::is_same_v<TOut, double> &&
                                      PROD == P_COMPACT)) {
       if (variant == 3) {

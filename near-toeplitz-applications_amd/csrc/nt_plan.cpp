@@ -144,42 +144,54 @@ int nt_plan_build(nt_plan* p, int64_t n, const double coeffs[7], char* err, int 
 
 namespace {
 
-// One block of the lane solve, as the kernel runs it: forward
+// One block of len rows of the lane solve, as the kernel runs it: forward
 // y_0 = d_0 ip_0, y_r = (d_r - a_r y_{r-1}) ip_r, then y_r -= cp_r y_{r+1}.
-// kind: 0 first block, 1 interior, 2 last.
-void lane_block(const NtLaneTab& t, int kind, const double* d, double* y) {
-  const int m = t.m;
+// kind: 0 first block, 1 interior, 2 last (its row len-1 has an and ip_last).
+void lane_block(const NtLaneTab& t, int kind, int len, const double* d, double* y) {
   const double* ip = t.ip[kind == 0 ? 0 : 1];
   const double* cp = t.cp[kind == 0 ? 0 : 1];
   y[0] = d[0] * ip[0];
-  for (int r = 1; r < m; ++r) {
-    const bool last = (kind == 2 && r == m - 1);
+  for (int r = 1; r < len; ++r) {
+    const bool last = (kind == 2 && r == len - 1);
     y[r] = (d[r] - (last ? t.an : t.ai) * y[r - 1]) * (last ? t.ip_last : ip[r]);
   }
-  for (int r = m - 2; r >= 0; --r) y[r] -= cp[r] * y[r + 1];
+  for (int r = len - 2; r >= 0; --r) y[r] -= cp[r] * y[r + 1];
+}
+
+int lane_len(const NtLaneTab& t, int p) { return p < t.j ? t.m : t.m - 1; }
+int lane_kind(int p) { return p == 0 ? 0 : (p == NT_LANE_P - 1 ? 2 : 1); }
+const double* lane_uh(const NtLaneTab& t, int p) {
+  return t.uh[p == NT_LANE_P - 1 ? 2 : (lane_len(t, p) == t.m ? 0 : 1)];
+}
+const double* lane_lh(const NtLaneTab& t, int p) {
+  return t.lh[p == 0 ? 0 : (lane_len(t, p) == t.m ? 1 : 2)];
 }
 
 // The whole lane solve of one line on the host (the kernel's arithmetic).
 void lane_solve(const NtLaneTab& t, const double* d, double* x) {
-  const int P = NT_LANE_P, m = t.m, NR = NT_LANE_NR;
+  const int P = NT_LANE_P, NR = NT_LANE_NR;
   double y[NT_LANE_P * NT_LANE_MAX_M], g[NT_LANE_NR];
-  for (int p = 0; p < P; ++p) lane_block(t, p == 0 ? 0 : (p == P - 1 ? 2 : 1), d + p * m, y + p * m);
+  int start[NT_LANE_P + 1];
+  start[0] = 0;
+  for (int p = 0; p < P; ++p) start[p + 1] = start[p] + lane_len(t, p);
+  for (int p = 0; p < P; ++p) lane_block(t, lane_kind(p), lane_len(t, p), d + start[p], y + start[p]);
   for (int i = 0; i < NR; ++i) {
     const int p = (i + 1) / 2;
-    const double r = (i % 2 == 1) ? -y[p * m] : -y[p * m + m - 1];
+    const double r = (i % 2 == 1) ? -y[start[p]] : -y[start[p + 1] - 1];
     g[i] = (r - (i ? t.rsub[i] * g[i - 1] : 0.0)) * t.rip[i];
   }
   for (int i = NR - 2; i >= 0; --i) g[i] -= t.rcp[i] * g[i + 1];
   for (int p = 0; p < P; ++p) {
     const double al = (p >= 1) ? g[2 * p - 1] : 0.0, be = (p <= P - 2) ? g[2 * p] : 0.0;
-    const double* uh = t.uh[p == P - 1 ? 1 : 0];
-    const double* lh = t.lh[p == 0 ? 0 : 1];
-    for (int r = 0; r < m; ++r) x[p * m + r] = y[p * m + r] + al * uh[r] + be * lh[r];
+    const double* uh = lane_uh(t, p);
+    const double* lh = lane_lh(t, p);
+    for (int r = 0; r < lane_len(t, p); ++r) x[start[p] + r] = y[start[p] + r] + al * uh[r] + be * lh[r];
   }
 }
 
 bool lane_off(NtLaneTab* t) {
   t->m = 0;
+  t->j = 0;
   return false;
 }
 
@@ -188,15 +200,16 @@ bool lane_off(NtLaneTab* t) {
 bool nt_lane_build(NtLaneTab* t, int64_t n, const double coeffs[7]) {
   std::memset(t, 0, sizeof(*t));
   const int P = NT_LANE_P, NR = NT_LANE_NR;
-  if (n % P != 0) return false;
-  const int m = static_cast<int>(n / P);
-  if (m != 8 && m != 16 && m != 32) return false;
+  if (n <= 3 * P || n > P * NT_LANE_MAX_M) return false;
+  const int m = static_cast<int>((n + P - 1) / P);
   const double b1 = coeffs[0], c1 = coeffs[1], ai = coeffs[2], bi = coeffs[3], ci = coeffs[4],
                an = coeffs[5], bn = coeffs[6];
   t->m = m;
+  t->j = static_cast<int>(n - static_cast<int64_t>(P) * (m - 1));
   t->ai = ai;
   t->an = an;
-  // block sweeps: the first block starts with (b1, c1), the others with (bi, ci)
+  // block sweeps: the first block starts with (b1, c1), the others with (bi, ci);
+  // a block of m - 1 rows uses the first m - 1 pivots of its kind
   for (int k = 0; k < 2; ++k) {
     double cprev = 0.0;
     for (int r = 0; r < m; ++r) {
@@ -208,32 +221,37 @@ bool nt_lane_build(NtLaneTab* t, int64_t n, const double coeffs[7]) {
       cprev = t->cp[k][r];
     }
   }
-  const double pl = bn - an * t->cp[1][m - 2];
+  const int llast = lane_len(*t, P - 1);
+  const double pl = bn - an * t->cp[1][llast - 2];
   if (!finite_nonzero(pl)) return lane_off(t);
   t->ip_last = 1.0 / pl;
-  // responses to the couplings: uh = A^-1 (-ai e_0), lh = A^-1 (-ci e_{m-1})
+  // responses to the couplings: uh = A^-1 (-ai e_0), lh = A^-1 (-ci e_{len-1})
   double e[NT_LANE_MAX_M] = {};
   e[0] = -ai;
-  lane_block(*t, 1, e, t->uh[0]);
-  lane_block(*t, 2, e, t->uh[1]);
+  lane_block(*t, 1, m, e, t->uh[0]);
+  lane_block(*t, 1, m - 1, e, t->uh[1]);
+  lane_block(*t, 2, llast, e, t->uh[2]);
   e[0] = 0.0;
   e[m - 1] = -ci;
-  lane_block(*t, 0, e, t->lh[0]);
-  lane_block(*t, 1, e, t->lh[1]);
+  lane_block(*t, 0, m, e, t->lh[0]);
+  lane_block(*t, 1, m, e, t->lh[1]);
+  e[m - 1] = 0.0;
+  e[m - 2] = -ci;
+  lane_block(*t, 1, m - 1, e, t->lh[2]);
   // reduced system, row i for z_{i+1}: odd i: first row of block (i+1)/2,
-  // x_p[0] = z_{2p-1}; even i: last row of block i/2, x_p[m-1] = z_{2p+2}
+  // x_p[0] = z_{2p-1}; even i: last row of block i/2, x_p[len-1] = z_{2p+2}
   double cprev = 0.0;
   for (int i = 0; i < NR; ++i) {
     double sub, dia, sup;
     if (i % 2 == 1) {
       const int p = (i + 1) / 2;
       sub = -1.0;
-      dia = t->uh[p == P - 1 ? 1 : 0][0];
-      sup = (p <= P - 2) ? t->lh[1][0] : 0.0;
+      dia = lane_uh(*t, p)[0];
+      sup = (p <= P - 2) ? lane_lh(*t, p)[0] : 0.0;
     } else {
-      const int p = i / 2;
-      sub = (p >= 1) ? t->uh[0][m - 1] : 0.0;
-      dia = t->lh[p == 0 ? 0 : 1][m - 1];
+      const int p = i / 2, len = lane_len(*t, p);
+      sub = (p >= 1) ? lane_uh(*t, p)[len - 1] : 0.0;
+      dia = lane_lh(*t, p)[len - 1];
       sup = -1.0;
     }
     const double piv = dia - (i ? sub * cprev : 0.0);

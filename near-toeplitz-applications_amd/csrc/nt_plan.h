@@ -59,9 +59,10 @@ struct NtPlanK {
 };
 
 // Block-partitioned solve with one lane per line (strided lines whose adjacent
-// lines are adjacent in memory): the n rows are cut into NT_LANE_P blocks of
-// m = n / NT_LANE_P rows, one wavefront per block, each lane a line.  Each block
-// is solved on its own by a Thomas sweep (y), then
+// lines are adjacent in memory): the n rows are cut into NT_LANE_P blocks, one
+// wavefront per block, each lane a line.  With m = ceil(n / NT_LANE_P), the
+// first j = n - NT_LANE_P (m - 1) blocks have m rows and the others m - 1.  Each
+// block is solved on its own by a Thomas sweep (y), then
 //   x = y + alpha * uh + beta * lh
 // with alpha the previous block's last value and beta the next block's first
 // (uh, lh: the block's responses to its two couplings, RHS-independent).  The
@@ -73,14 +74,16 @@ struct NtPlanK {
 #define NT_LANE_MAX_M 32
 #define NT_LANE_NR (2 * NT_LANE_P - 2)
 struct NtLaneTab {
-  int32_t m;                        // rows per block; 0: no lane solve for this plan
-  int32_t pad_;
+  int32_t m;                        // rows of the long blocks; 0: no lane solve for this plan
+  int32_t j;                        // blocks 0..j-1 have m rows, blocks j..P-1 have m - 1
   double ai, an;                    // sub-diagonal of block rows >= 1 (an: last row of the last block)
   double ip[2][NT_LANE_MAX_M];      // 1 / pivot of the block sweep: [0] first block, [1] the others
   double cp[2][NT_LANE_MAX_M];      // super-diagonal / pivot (the last block shares [1])
   double ip_last;                   // 1 / pivot of the last block's last row
-  double uh[2][NT_LANE_MAX_M];      // response to alpha: [0] interior blocks, [1] last block
-  double lh[2][NT_LANE_MAX_M];      // response to beta: [0] first block, [1] interior blocks
+  // response to alpha: [0] interior blocks of m rows, [1] of m - 1 rows, [2] the last block
+  double uh[3][NT_LANE_MAX_M];
+  // response to beta: [0] the first block, [1] interior blocks of m rows, [2] of m - 1 rows
+  double lh[3][NT_LANE_MAX_M];
   double rsub[NT_LANE_NR];          // reduced system: sub-diagonal,
   double rip[NT_LANE_NR];           //   1 / pivot,
   double rcp[NT_LANE_NR];           //   super-diagonal / pivot
@@ -100,7 +103,8 @@ struct nt_plan {
 // with a message in `err` (zero pivot, n out of range, non-finite input).
 int nt_plan_build(nt_plan* p, int64_t n, const double coeffs[7], char* err, int errlen);
 
-// Fill t for n = NT_LANE_P * m (m in 8, 16, 32) and the matrix coeffs; returns
-// false (t->m = 0) for other n, a zero pivot, or when the block solve of a test
-// vector differs from a whole-system Thomas solve by more than 1e-11 relative.
+// Fill t for 3 NT_LANE_P < n <= NT_LANE_P NT_LANE_MAX_M (blocks of at least 3
+// rows) and the matrix coeffs; returns false (t->m = 0) for other n, a zero
+// pivot, or when the block solve of a test vector differs from a whole-system
+// Thomas solve by more than 1e-11 relative.
 bool nt_lane_build(NtLaneTab* t, int64_t n, const double coeffs[7]);

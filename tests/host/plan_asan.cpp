@@ -10,7 +10,7 @@
 //   * every factor is finite and the FP32 table is the FP64 one rounded;
 //   * diagonal dominance of the interior rows survives every reduction for
 //     the diagonally dominant families (cyclic reduction preserves it);
-//   * the lane-solve tables (n = 128, 256, 512): every block response solves
+//   * the lane-solve tables (49 <= n <= 512): every block response solves
 //     its block equations, and the lane kernel's arithmetic restated here
 //     matches a band elimination with partial pivoting of the whole matrix;
 //     other n, and a zero block pivot, leave no lane tables.
@@ -108,29 +108,37 @@ std::vector<double> gepp_solve(long long n, const double co[7], std::vector<doub
   return x;
 }
 
-// The lane kernel's arithmetic for one line: block sweeps, the reduced
-// system's forward and back sweeps, the correction.
+// The lane kernel's arithmetic for one line: block sweeps (blocks 0..j-1 of
+// m rows, the others of m - 1), the reduced system's forward and back sweeps,
+// the correction.
 std::vector<double> lane_kernel_host(const NtLaneTab& t, const std::vector<double>& rhs) {
   const int P = NT_LANE_P, m = t.m, NR = NT_LANE_NR;
   std::vector<double> y(rhs), red(NR);
+  std::vector<int> start(P + 1, 0), len(P);
   for (int w = 0; w < P; ++w) {
-    double* v = &y[w * m];
+    len[w] = (w < t.j) ? m : m - 1;
+    start[w + 1] = start[w] + len[w];
+  }
+  for (int w = 0; w < P; ++w) {
+    double* v = &y[start[w]];
+    const int L = len[w];
     const double* ip = t.ip[w == 0 ? 0 : 1];
     const double* cp = t.cp[w == 0 ? 0 : 1];
     v[0] *= ip[0];
-    for (int r = 1; r < m - 1; ++r) v[r] = (v[r] - t.ai * v[r - 1]) * ip[r];
-    v[m - 1] = (v[m - 1] - (w == P - 1 ? t.an : t.ai) * v[m - 2]) * (w == P - 1 ? t.ip_last : ip[m - 1]);
-    for (int r = m - 2; r >= 0; --r) v[r] -= cp[r] * v[r + 1];
+    for (int r = 1; r < L - 1; ++r) v[r] = (v[r] - t.ai * v[r - 1]) * ip[r];
+    v[L - 1] = (v[L - 1] - (w == P - 1 ? t.an : t.ai) * v[L - 2]) * (w == P - 1 ? t.ip_last : ip[L - 1]);
+    for (int r = L - 2; r >= 0; --r) v[r] -= cp[r] * v[r + 1];
     if (w >= 1) red[2 * w - 1] = -v[0];
-    if (w <= P - 2) red[2 * w] = -v[m - 1];
+    if (w <= P - 2) red[2 * w] = -v[L - 1];
   }
   red[0] *= t.rip[0];
   for (int i = 1; i < NR; ++i) red[i] = (red[i] - t.rsub[i] * red[i - 1]) * t.rip[i];
   for (int i = NR - 2; i >= 0; --i) red[i] -= t.rcp[i] * red[i + 1];
   for (int w = 0; w < P; ++w) {
     const double al = (w >= 1) ? red[2 * w - 1] : 0.0, be = (w <= P - 2) ? red[2 * w] : 0.0;
-    for (int r = 0; r < m; ++r)
-      y[w * m + r] += al * t.uh[w == P - 1 ? 1 : 0][r] + be * t.lh[w == 0 ? 0 : 1][r];
+    const double* uh = t.uh[w == P - 1 ? 2 : (len[w] == m ? 0 : 1)];
+    const double* lh = t.lh[w == 0 ? 0 : (len[w] == m ? 1 : 2)];
+    for (int r = 0; r < len[w]; ++r) y[start[w] + r] += al * uh[r] + be * lh[r];
   }
   return y;
 }
@@ -153,24 +161,27 @@ double block_residual(int m, double bf, double cf, double ai, double bi, double 
 
 void check_lane(const nt_plan& p, long long n, const double co[7], bool expect) {
   const NtLaneTab& t = p.lane;
-  const bool size_ok = (n == 128 || n == 256 || n == 512);
+  const bool size_ok = (n > 3 * NT_LANE_P && n <= NT_LANE_P * NT_LANE_MAX_M);
   if (!(size_ok && expect)) {
     if (t.m != 0) fail("no lane tables expected", n);
     return;
   }
-  if (t.m * NT_LANE_P != n) {
-    fail("lane tables built", n);
+  const int m = static_cast<int>((n + NT_LANE_P - 1) / NT_LANE_P);
+  if (t.m != m || t.j != n - NT_LANE_P * (m - 1) || t.j < 1 || t.j > NT_LANE_P) {
+    fail("lane tables built with m = ceil(n / 16) and j long blocks", n);
     return;
   }
-  const int m = t.m;
   const double b1 = co[0], c1 = co[1], ai = co[2], bi = co[3], ci = co[4], an = co[5], bn = co[6];
   const double scale = std::fabs(ai) + std::fabs(ci) + 1e-300;
-  // uh: the interior and last blocks' response to -ai on row 0; lh: the first
-  // and interior blocks' response to -ci on row m-1
+  const int mlast = (t.j == NT_LANE_P) ? m : m - 1;
+  // uh: the interior (m and m - 1 rows) and last blocks' response to -ai on
+  // row 0; lh: the first and interior blocks' response to -ci on their last row
   if (block_residual(m, bi, ci, ai, bi, ci, ai, bi, t.uh[0], -ai, 0.0) > 1e-14 * scale ||
-      block_residual(m, bi, ci, ai, bi, ci, an, bn, t.uh[1], -ai, 0.0) > 1e-14 * scale ||
+      block_residual(m - 1, bi, ci, ai, bi, ci, ai, bi, t.uh[1], -ai, 0.0) > 1e-14 * scale ||
+      block_residual(mlast, bi, ci, ai, bi, ci, an, bn, t.uh[2], -ai, 0.0) > 1e-14 * scale ||
       block_residual(m, b1, c1, ai, bi, ci, ai, bi, t.lh[0], 0.0, -ci) > 1e-14 * scale ||
-      block_residual(m, bi, ci, ai, bi, ci, ai, bi, t.lh[1], 0.0, -ci) > 1e-14 * scale)
+      block_residual(m, bi, ci, ai, bi, ci, ai, bi, t.lh[1], 0.0, -ci) > 1e-14 * scale ||
+      block_residual(m - 1, bi, ci, ai, bi, ci, ai, bi, t.lh[2], 0.0, -ci) > 1e-14 * scale)
     fail("block responses solve their block equations", n);
   std::vector<double> rhs(n);
   for (long long i = 0; i < n; ++i) rhs[i] = std::cos(1.3 * i) - 0.5 * std::sin(0.37 * i * i);

@@ -1,10 +1,11 @@
 """The block-partitioned lane solve (strided variant 5) against the oracle.
 
-Strided lines of 128, 256 and 512 points whose neighbouring lines are
-neighbours in memory (y- and z-lines of C-ordered fields) are solved with one
-lane per line: sixteen wavefronts each take one 16th of every line, solve their
-block by a Thomas sweep, and the blocks are joined through the tridiagonal
-reduced system of the reference's partition method
+Strided lines of 49 to 512 points whose neighbouring lines are neighbours in
+memory (y- and z-lines of C-ordered fields) are solved with one lane per line:
+sixteen wavefronts each take one block of every line (m = ceil(n / 16) rows for
+the first n - 16 (m - 1) blocks, m - 1 for the others), solve their block by a
+Thomas sweep, and the blocks are joined through the tridiagonal reduced system
+of the reference's partition method
 (parallelcompact/reduced.py:163-178).  The results must match the SciPy banded
 oracle to the FP64 tolerance (1e-12 relative) for the plain solve
 (neato.py:168-197 on permuted lines) and the fused compact derivative
@@ -48,7 +49,7 @@ def _with_variant(n, v, fn):
 
 
 @pytest.mark.parametrize("name", sorted(COEFFS))
-@pytest.mark.parametrize("n", [128, 256, 512])
+@pytest.mark.parametrize("n", [49, 100, 128, 300, 497, 512])
 @pytest.mark.parametrize("axis", [0, 1])
 def test_lane_solve_axis_in_place(dev, name, n, axis):
     from ntamd import Plan, core
@@ -68,7 +69,7 @@ def test_lane_solve_axis_in_place(dev, name, n, axis):
     assert np.abs(got - run(2)).max() / np.abs(want).max() < 1e-13
 
 
-@pytest.mark.parametrize("n", [128, 256, 512])
+@pytest.mark.parametrize("n", [52, 64, 96, 100, 128, 200, 256, 300, 384, 480, 511, 512])
 @pytest.mark.parametrize("axis", [0, 1])
 def test_lane_compact_derivative(dev, n, axis):
     from ntamd import COMPACT_COEFFS, Plan, core
