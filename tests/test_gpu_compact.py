@@ -105,19 +105,29 @@ def test_fp32_storage_strided_tiles(dev, shape, axis, precision):
     assert rel_err(out.cpu().numpy(), want) < tol
 
 
+@pytest.mark.parametrize("variant", [None, 2])
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
 @pytest.mark.parametrize("shape", [(512, 32, 8192), (512, 64, 8192)])
-def test_fp32_storage_z_lines_far_rows(dev, shape, precision):
-    """FP32-stored z-lines with rows 1 MiB apart (the fast tile path) and 2 MiB
-    apart (the general one), 262,144 and 524,288 lines: 256 lines drawn at
-    random are checked against the oracle (lines are independent)."""
+def test_fp32_storage_z_lines_far_rows(dev, shape, precision, variant):
+    """FP32-stored z-lines with rows 1 MiB and 2 MiB apart, 262,144 and 524,288
+    lines, with the default launch shape (the FP64 solve: one lane per line)
+    and with the 32-line FP32 tile kernels (variant 2: rows 1 MiB apart take
+    the fast tile path, 2 MiB apart the general one; FP32 + refinement takes
+    them by default): 256 lines drawn at random are checked against the oracle
+    (lines are independent)."""
     from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.autotune import _set, current
     from ntamd.workloads import compact_field_device
     n = shape[0]
     h = 1.0 / (n - 1)
     fd = compact_field_device(shape, seed=11, dtype=torch.float32, device=dev)
     out = torch.full(shape, float("nan"), dtype=torch.float64, device=dev)
-    core.compact_derivative(Plan(n, COMPACT_COEFFS), fd, out, h, 0, precision, 1)
+    keep, _ = current(1, n, precision)
+    _set(1, precision, n, keep if variant is None else variant)
+    try:
+        core.compact_derivative(Plan(n, COMPACT_COEFFS), fd, out, h, 0, precision, 1)
+    finally:
+        _set(1, precision, n, keep)
     rng = np.random.default_rng(3)
     ys = torch.tensor(rng.integers(0, shape[1], 256), device=dev)
     xs = torch.tensor(rng.integers(0, shape[2], 256), device=dev)
@@ -318,17 +328,27 @@ def test_512cubed_properties(dev, axis):
         assert rel_err(out[tuple(sl)].cpu().numpy(), want) < FP64_TOL
 
 
+@pytest.mark.parametrize("variant", [None, 2])
 @pytest.mark.parametrize("shape", [(512, 64, 128), (512, 33, 250)])
-def test_z_lines_far_rows(dev, shape):
-    """z-lines whose rows are 64 KiB or more apart: 8192 lines (whole aligned
-    tiles, the fast path of the whole-tile kernel) and 8250 lines (a ragged last
-    tile, the general kernel).  Both equal the oracle."""
+def test_z_lines_far_rows(dev, shape, variant):
+    """z-lines whose rows are 64 KiB or more apart: 8192 lines and 8250 lines,
+    with the default launch shape (one lane per line, 128 groups of 64 lines
+    and a ragged group of 58) and with the whole-tile kernel (variant 2: whole
+    aligned tiles take its fast path, the ragged last tile the general kernel).
+    All equal the oracle."""
     from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.autotune import _set, current
     from ntamd.workloads import compact_field
     f = compact_field(shape, seed=7)
-    h = 1.0 / (shape[0] - 1)
+    n = shape[0]
+    h = 1.0 / (n - 1)
     want = compact.derivative_banded(f, h, 0)
     fd = torch.tensor(f, device=dev)
     out = torch.full_like(fd, float("nan"))
-    core.compact_derivative(Plan(shape[0], COMPACT_COEFFS), fd, out, h, 0, "fp64")
+    keep, _ = current(1, n, "fp64")
+    _set(1, "fp64", n, keep if variant is None else variant)
+    try:
+        core.compact_derivative(Plan(n, COMPACT_COEFFS), fd, out, h, 0, "fp64")
+    finally:
+        _set(1, "fp64", n, keep)
     assert np.abs(out.cpu().numpy() - want).max() / np.abs(want).max() < 1e-12
