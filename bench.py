@@ -17,7 +17,8 @@ sweeps ("refine") on the headline's field, timed before the headline; ADI
 1024^3 field stored in FP32 ("fp32_storage"), 64 mixed-size grids in one
 persistent launch ("batch"), and a Poisson request stream served with
 continuous batching ("serving"), timed after it.  --quick skips them; --case
-runs only the named ones.
+runs only the named ones.  "strided_sizes" (d/dz and d/dy of 2^26-point fields
+with 96..480-point lines) runs only when named.
 
 Every result's error is measured against the CPU oracle (SciPy banded solve
 of the same matrix, oracle/) on lines copied from the same device field, with
@@ -389,6 +390,30 @@ def pmc_probe():
     print("pmc probe done", flush=True)
 
 
+def strided_sizes_case(steps, warmup, sizes=(96, 160, 300, 384, 480)):
+    """d/dz and d/dy of about 2^26-point FP64 fields whose lines are not a power
+    of two long, with the default launch shapes (one lane per line): per length
+    [us per launch, GB/s at 16 B/point, max rel err vs the oracle on 256 lines]."""
+    import torch
+    from ntamd.workloads import compact_field
+    res = {}
+    for n in sizes:
+        m = int(round(((1 << 26) / n) ** 0.5))
+        m -= m % 64
+        for axis, name in ((0, "ddz"), (1, "ddy")):
+            shape = (n, m, m) if axis == 0 else (m, n, m)
+            f = torch.from_numpy(compact_field(shape, seed=1)).to("cuda")
+            out = torch.empty_like(f)
+            t, _ = compact_case(f, axis, "fp64", 1, steps, warmup, out=out)
+            err = oracle_err(f, out, axis, 1.0 / (n - 1), 256, seed=n + axis)
+            res[f"{name}_{n}"] = [t["mean_s"] * 1e6, t["gbps"], err]
+            log(f"[bench] strided {name} n={n} {shape}: {t['mean_s'] * 1e6:.1f} us, "
+                f"{t['gbps']:.0f} GB/s, err {err:.2e}")
+            del f, out
+            torch.cuda.empty_cache()
+    return res
+
+
 def config4_case(N=1024, reps=5):
     """1024^3 field stored in FP32 (4 GiB): d/dx, d/dy, d/dz with the FP32 solve +
     one FP64-residual sweep, against plain FP64 (the FP64-stored field, FP64
@@ -574,6 +599,8 @@ def compact_served(served):
 
 
 EXTRA_CASES = ("axes", "refine", "adi", "fp32_storage", "batch", "serving")
+# run only when named with --case
+OPTIONAL_CASES = ("strided_sizes",)
 EXTRA_WARMUP = 30          # untimed launches of each extra 512^3 case
 
 
@@ -586,7 +613,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short program profiled for HBM traffic and exit")
-    ap.add_argument("--case", action="append", choices=EXTRA_CASES,
+    ap.add_argument("--case", action="append", choices=EXTRA_CASES + OPTIONAL_CASES,
                     help="run only these extra cases (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -676,6 +703,8 @@ def main():
                                                             lines_per_grid=8)
     if "serving" in cases:
         extras["serving"] = serving_case()
+    if "strided_sizes" in cases:
+        extras["strided_sizes"] = strided_sizes_case(short, xw)
 
     pts = N ** 3
     rec = {
