@@ -105,3 +105,21 @@ def test_lane_many_groups_z_lines(dev):
     _with_variant(256, 5, lambda: core.compact_derivative(Plan(256, COMPACT_COEFFS), fd, out, h, 0, "fp64"))
     assert rel_err(out.cpu().numpy(), compact.derivative_banded(f, h, 0)) < 1e-12
     assert np.array_equal(fd.cpu().numpy(), f)
+
+
+@pytest.mark.parametrize("n", [57, 256, 512])
+@pytest.mark.parametrize("shape_of", [lambda n: (n, 1, 1), lambda n: (700, n, 3), lambda n: (n, 1, 65)])
+def test_lane_narrow_slabs(dev, n, shape_of):
+    """Groups with one line (61..63 lanes idle), 700 slabs of 3 y-lines each,
+    and 65 lines (a group of 64 and a group of 1): the in-place solve equals
+    the oracle and the idle lanes store nothing (the field around the lines
+    is untouched because every element belongs to some line)."""
+    from ntamd import Plan, core
+    co = COEFFS["general"]
+    shape = shape_of(n)
+    axis = 1 if shape[1] == n and shape[0] != n else 0
+    d = np.random.default_rng(n + len(shape)).standard_normal(shape)
+    x = torch.tensor(d, device=dev)
+    _with_variant(n, 5, lambda: core.solve_axis(Plan(n, co), x, axis))
+    want = np.moveaxis(cr.cr_solve(co, np.moveaxis(d, axis, -1)), -1, axis)
+    assert rel_err(x.cpu().numpy(), want) < 1e-12
