@@ -181,4 +181,175 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes(LaneArgs arg
   }
 }
 
+// FP32-stored compact derivative with the FP32 solve and one FP64-residual
+// refinement sweep, one lane per line, every block full (n = 16 M).  A lane
+// keeps its block's FP32 field values and FP32 solution in registers:
+//   1. load the FP32 values; the block's end rows through LDS; barrier;
+//   2. x = FP32(d), d the compact right-hand side formed in FP64
+//      (compact/rhs.py:21-36, one-sided closures at the line ends);
+//   3. FP32 block sweep, reduced system (wavefront 0, LDS), correction -> x;
+//   4. x's end rows through LDS; r = FP32(d - A x) formed in FP64, each row
+//      written over the field value that no later row needs;
+//   5. the same FP32 solve of r -> e;
+//   6. out = x + e in FP64.
+// Steps 2 and 4 are the arithmetic of the general FP32 + refinement path
+// (compact_solve_from_f32 in nt_line_kernel.h); the solves are the lane solve
+// in FP32 with the plan's factors rounded to FP32.
+struct LaneArgs32 {
+  const float* in;
+  double* out;
+  int64_t estride, ostride, phys, groups;
+  double s1, s2;
+  double b1, c1, ai, bi, ci, an, bn;   // the FP64 matrix of the residual
+  NtLaneTab32 t;
+};
+
+__device__ __forceinline__ const LaneArgs32& lane32_args() {
+  return *(const LaneArgs32*)(__builtin_amdgcn_kernarg_segment_ptr());
+}
+
+template <int M>
+__global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArgs32 args_in) {
+  (void)args_in;
+  static_assert(M >= 4 && M <= NT_LANE_MAX_M, "rows per block");
+  constexpr int NW = NT_LANE_P, NR = NT_LANE_NR;
+  // unrolled by two while that keeps two workgroups per CU (at most 64 VGPRs:
+  // 13 rows per lane), fully above (see line_kernel_lanes)
+  constexpr int RED_UNROLL = (M <= 13) ? 2 : NR;
+  const LaneArgs32& A = lane32_args();
+  __shared__ float edge[2][NW][64];
+  __shared__ float red[NR][64];
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  const bool lastw = (w == NW - 1);
+  const int64_t slab = static_cast<int64_t>(blockIdx.x) / A.groups;
+  const int64_t g0 = (static_cast<int64_t>(blockIdx.x) - slab * A.groups) * 64;
+  const int64_t left = A.phys - g0;   // lines in this group: 1..64 used
+  const bool on = lane < left;
+  const int64_t lc = on ? lane : left - 1;   // lanes past the last line re-read it, store nothing
+  const int64_t row0 = slab * A.ostride + g0 + static_cast<int64_t>(w) * M * A.estride;
+  const int64_t es = A.estride;
+
+  // 1. load
+  float f[M];
+  {
+    const float* src = A.in + row0 + lc;
+#pragma unroll
+    for (int r = 0; r < M; ++r) f[r] = __builtin_nontemporal_load(src + r * es);
+  }
+  edge[0][w][lane] = f[0];
+  edge[1][w][lane] = f[M - 1];
+  __syncthreads();
+  const float fleft = (w > 0) ? edge[1][w > 0 ? w - 1 : 0][lane] : 0.0f;
+  const float fright = (w < NW - 1) ? edge[0][w < NW - 1 ? w + 1 : 0][lane] : 0.0f;
+  const double s1 = A.s1, s2 = A.s2;
+  const double b0 = (w == 0) ? s2 * (-5.0 * static_cast<double>(f[0]) + 4.0 * static_cast<double>(f[1]) +
+                                     static_cast<double>(f[2]))
+                             : 0.0;
+  const double bn = lastw ? -s2 * (-5.0 * static_cast<double>(f[M - 1]) + 4.0 * static_cast<double>(f[M - 2]) +
+                                   static_cast<double>(f[M - 3]))
+                          : 0.0;
+
+  // 2. x = FP32(d)
+  float x[M];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const float fm = (r > 0) ? f[r > 0 ? r - 1 : 0] : fleft;
+    const float fp = (r < M - 1) ? f[r < M - 1 ? r + 1 : 0] : fright;
+    double d = s1 * (static_cast<double>(fp) - static_cast<double>(fm));
+    if (r == 0 && w == 0) d = b0;
+    if (r == M - 1 && lastw) d = bn;
+    x[r] = static_cast<float>(d);
+  }
+
+  const float* ip = A.t.ip[w == 0 ? 0 : 1];
+  const float* cp = A.t.cp[w == 0 ? 0 : 1];
+  const float* uh = A.t.uh[lastw ? 1 : 0];
+  const float* lh = A.t.lh[w == 0 ? 0 : 1];
+  const float ai = A.t.ai;
+  const float al = lastw ? A.t.an : ai;
+  const float il = lastw ? A.t.ip_last : ip[M - 1];
+  // the reduced system of the 64 lines: this block's first and last values
+  // in, alpha (previous block's last) and beta (next block's first) out
+  auto reduced = [&](float first, float last, float& alpha, float& beta) {
+    if (w >= 1) red[2 * w - 1][lane] = -first;
+    if (w <= NW - 2) red[2 * w][lane] = -last;
+    __syncthreads();
+    if (w == 0) {
+      float g = red[0][lane] * A.t.rip[0];
+      red[0][lane] = g;
+#pragma unroll RED_UNROLL
+      for (int i = 1; i < NR; ++i) {
+        g = (red[i][lane] - A.t.rsub[i] * g) * A.t.rip[i];
+        red[i][lane] = g;
+      }
+#pragma unroll RED_UNROLL
+      for (int i = NR - 2; i >= 0; --i) {
+        g = red[i][lane] - A.t.rcp[i] * g;
+        red[i][lane] = g;
+      }
+    }
+    __syncthreads();
+    alpha = (w >= 1) ? red[w >= 1 ? 2 * w - 1 : 0][lane] : 0.0f;
+    beta = (w <= NW - 2) ? red[w <= NW - 2 ? 2 * w : 0][lane] : 0.0f;
+  };
+  // FP32 block sweep of v in place, then the reduced system and the correction
+  auto solve32 = [&](float (&v)[M]) {
+    v[0] *= ip[0];
+#pragma unroll
+    for (int r = 1; r < M - 1; ++r) v[r] = (v[r] - ai * v[r - 1]) * ip[r];
+    v[M - 1] = (v[M - 1] - al * v[M - 2]) * il;
+#pragma unroll
+    for (int r = M - 2; r >= 0; --r) v[r] -= cp[r] * v[r + 1];
+    float alpha, beta;
+    reduced(v[0], v[M - 1], alpha, beta);
+#pragma unroll
+    for (int r = 0; r < M; ++r) v[r] += alpha * uh[r] + beta * lh[r];
+  };
+
+  // 3. FP32 solve of x
+  solve32(x);
+
+  // 4. residual in FP64, over the field values (the edge rows were read before
+  // the reduced system's first barrier, so they can be reused)
+  edge[0][w][lane] = x[0];
+  edge[1][w][lane] = x[M - 1];
+  __syncthreads();
+  {
+    const float xl = (w > 0) ? edge[1][w > 0 ? w - 1 : 0][lane] : 0.0f;
+    const float xr = (w < NW - 1) ? edge[0][w < NW - 1 ? w + 1 : 0][lane] : 0.0f;
+    const double b1 = A.b1, c1 = A.c1, aI = A.ai, bI = A.bi, cI = A.ci, aN = A.an, bN = A.bn;
+    float pending = 0.0f;
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const float fm = (r > 0) ? f[r > 0 ? r - 1 : 0] : fleft;
+      const float fp = (r < M - 1) ? f[r < M - 1 ? r + 1 : 0] : fright;
+      double d = s1 * (static_cast<double>(fp) - static_cast<double>(fm));
+      if (r == 0 && w == 0) d = b0;
+      if (r == M - 1 && lastw) d = bn;
+      const double xm = static_cast<double>((r > 0) ? x[r > 0 ? r - 1 : 0] : xl);
+      const double xc = static_cast<double>(x[r]);
+      const double xp = static_cast<double>((r < M - 1) ? x[r < M - 1 ? r + 1 : 0] : xr);
+      double ax = aI * xm + bI * xc + cI * xp;
+      if (r == 0 && w == 0) ax = b1 * xc + c1 * xp;
+      if (r == M - 1 && lastw) ax = aN * xm + bN * xc;
+      // row r-1's field value is used for the last time by row r
+      if (r > 0) f[r > 0 ? r - 1 : 0] = pending;
+      pending = static_cast<float>(d - ax);
+    }
+    f[M - 1] = pending;
+  }
+
+  // 5. FP32 solve of r -> e, in f
+  solve32(f);
+
+  // 6. out = x + e
+  if (on) {
+    double* dst = A.out + row0 + lc;
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+      __builtin_nontemporal_store(static_cast<double>(x[r]) + static_cast<double>(f[r]), dst + r * es);
+  }
+}
+
 }  // namespace nt

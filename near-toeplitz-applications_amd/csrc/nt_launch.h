@@ -47,7 +47,9 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //           5: one lane per line, 64 lines per workgroup of 16 wavefronts,
 //              block-partitioned FP64 solve (49- to 512-point lines with
 //              adjacent lines adjacent in memory, FP64 or FP32 storage, outside
-//              ADI; otherwise as 1 up to 48 points and as 4 above)
+//              ADI); with FP32 + refinement, the FP32 block solve of FP32-stored
+//              compact derivatives with one sweep (n = 16 m, 64..512 points);
+//              other lines take lane_fallback()'s shape
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 // launch-shape variants per tuning layout: 0 contiguous, 1 strided, 2 and 3 the
 // same for the ADI half steps (their right-hand side reads neighbouring lines,
@@ -284,6 +286,57 @@ hipError_t launch_lanes(const LaneArgs& la, int m, unsigned grid, hipStream_t st
     if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
     return launch_lanes<TIn, PROD>(la_, a.lane->m, static_cast<unsigned>(grid_), st);      \
   } while (0)
+
+// FP32 + refinement lane solve of FP32-stored compact derivatives (STRIDED
+// variant 5, nt_lane_kernel.h), every block full: the instance for m rows.
+template <int M = 4>
+hipError_t launch_lanes_f32r(const LaneArgs32& la, int m, unsigned grid, hipStream_t st) {
+  if constexpr (M > NT_LANE_MAX_M) {
+    return hipErrorInvalidValue;
+  } else {
+    if (m == M) {
+      hipLaunchKernelGGL((line_kernel_lanes_f32r<M>), dim3(grid), dim3(NT_LANE_P * 64), 0, st, la);
+      return hipGetLastError();
+    }
+    return launch_lanes_f32r<M + 1>(la, m, grid, st);
+  }
+}
+
+#define NT_GOL32()                                                                         \
+  do {                                                                                     \
+    LaneArgs32 la_;                                                                        \
+    la_.in = static_cast<const float*>(a.in);                                              \
+    la_.out = static_cast<double*>(a.out);                                                 \
+    la_.estride = a.estride;                                                               \
+    la_.ostride = a.ostride;                                                               \
+    la_.phys = a.phys;                                                                     \
+    la_.groups = (a.phys + 63) / 64;                                                       \
+    la_.s1 = a.s1;                                                                         \
+    la_.s2 = a.s2;                                                                         \
+    la_.b1 = a.plan.b1;                                                                    \
+    la_.c1 = a.plan.c1;                                                                    \
+    la_.ai = a.plan.ai;                                                                    \
+    la_.bi = a.plan.bi;                                                                    \
+    la_.ci = a.plan.ci;                                                                    \
+    la_.an = a.plan.an;                                                                    \
+    la_.bn = a.plan.bn;                                                                    \
+    la_.t = *a.lane32;                                                                     \
+    const int64_t grid_ = outer * la_.groups;                                              \
+    if (grid_ <= 0) return hipSuccess;                                                     \
+    if (grid_ > 0x7fffffffLL) return hipErrorInvalidValue;                                 \
+    return launch_lanes_f32r(la_, a.lane32->m, static_cast<unsigned>(grid_), st);          \
+  } while (0)
+
+// Variant 5 for lines the lane kernels do not take: the shapes that are the
+// defaults without it (FP64: 32-line tiles up to 48 points, else the
+// segmented tiles; FP32 + refinement: the segmented tiles up to 256 points,
+// at 257..512 two lines per wavefront, or the whole-tile kernel for 512)
+inline int lane_fallback(int prec, int n) {
+  if (prec == PREC_F64) return (n <= 3 * NT_LANE_P) ? 1 : 4;
+  if (n <= 64) return (n == 64) ? 1 : 4;
+  if (n <= 256) return 4;
+  return ((n & (n - 1)) == 0) ? 2 : 3;
+}
 
 // Persistent whole-tile kernel, two lines per wavefront solved at once on
 // 32-lane segments (STRIDED variant 4).
@@ -538,10 +591,14 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     if constexpr (std::is_same_v<TOut, double> && PROD != P_ADI && PREC == PREC_F64) {
       if (variant == 5 && a.lane && a.lane->m >= 4 && a.pstride == 1) NT_GOL();
     }
-    // lines the lane kernel does not take: 32-line tiles below 49 points (33..48
-    // points, 2^26-point fields: 441-613 us against 523-744 for the segments),
-    // otherwise the segments
-    if (variant == 5) variant = (n <= 3 * NT_LANE_P) ? 1 : 4;
+    if constexpr (std::is_same_v<TIn, float> && std::is_same_v<TOut, double> && PROD == P_COMPACT &&
+                  PREC == PREC_F32R) {
+      if (variant == 5 && a.lane32 && a.pstride == 1 && a.sweeps == 1) NT_GOL32();
+    }
+    // lines the lane kernels do not take (FP64 below 49 points: 32-line tiles,
+    // 33..48 points on 2^26-point fields 441-613 us against 523-744 for the
+    // segments)
+    if (variant == 5) variant = lane_fallback(PREC, n);
     if constexpr (PROD != P_ADI && !(std::is_same_v<TIn, float> && std::is_same_v<TOut, double> &&
                                      PROD == P_COMPACT)) {
       if (variant == 3) {

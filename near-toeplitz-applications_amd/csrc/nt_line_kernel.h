@@ -43,6 +43,7 @@ struct LineArgs {
   int32_t sweeps;      // refinement sweeps (PREC_F32R)
   int32_t pad_;
   const NtLaneTab* lane;   // host copy of the plan's lane-solve tables or null (never read on the device)
+  const NtLaneTab32* lane32;   // their FP32 form (every block full), or null
 };
 
 // The launch arguments read in place from the kernel-argument segment (the

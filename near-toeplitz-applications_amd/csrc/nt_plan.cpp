@@ -139,6 +139,7 @@ int nt_plan_build(nt_plan* p, int64_t n, const double coeffs[7], char* err, int 
   fill_top(p->k.d.top, s);
   fill_top(p->k.f.top, s);
   nt_lane_build(&p->lane, n, coeffs);
+  nt_lane_build32(&p->lane32, p->lane);
   return 0;
 }
 
@@ -282,5 +283,44 @@ bool nt_lane_build(NtLaneTab* t, int64_t n, const double coeffs[7]) {
     xmax = std::fmax(xmax, std::fabs(dw[i]));
   }
   if (!(emax <= 1e-11 * xmax)) return lane_off(t);
+  return true;
+}
+
+bool nt_lane_build32(NtLaneTab32* t, const NtLaneTab& d) {
+  std::memset(t, 0, sizeof(*t));
+  if (d.m < 4 || d.j != NT_LANE_P) return false;
+  const int m = d.m;
+  t->ai = static_cast<float>(d.ai);
+  t->an = static_cast<float>(d.an);
+  t->ip_last = static_cast<float>(d.ip_last);
+  // every block full: interior blocks use uh[0] and lh[1], the last block
+  // uh[2], the first block lh[0]
+  for (int r = 0; r < m; ++r) {
+    for (int k = 0; k < 2; ++k) {
+      t->ip[k][r] = static_cast<float>(d.ip[k][r]);
+      t->cp[k][r] = static_cast<float>(d.cp[k][r]);
+    }
+    t->uh[0][r] = static_cast<float>(d.uh[0][r]);
+    t->uh[1][r] = static_cast<float>(d.uh[2][r]);
+    t->lh[0][r] = static_cast<float>(d.lh[0][r]);
+    t->lh[1][r] = static_cast<float>(d.lh[1][r]);
+  }
+  for (int i = 0; i < NT_LANE_NR; ++i) {
+    t->rsub[i] = static_cast<float>(d.rsub[i]);
+    t->rip[i] = static_cast<float>(d.rip[i]);
+    t->rcp[i] = static_cast<float>(d.rcp[i]);
+  }
+  bool ok = std::isfinite(t->ai) && std::isfinite(t->an) && std::isfinite(t->ip_last);
+  for (int r = 0; r < m; ++r)
+    for (int k = 0; k < 2; ++k)
+      ok = ok && std::isfinite(t->ip[k][r]) && std::isfinite(t->cp[k][r]) && std::isfinite(t->uh[k][r]) &&
+           std::isfinite(t->lh[k][r]);
+  for (int i = 0; i < NT_LANE_NR; ++i)
+    ok = ok && std::isfinite(t->rsub[i]) && std::isfinite(t->rip[i]) && std::isfinite(t->rcp[i]);
+  if (!ok) {
+    std::memset(t, 0, sizeof(*t));
+    return false;
+  }
+  t->m = m;
   return true;
 }
