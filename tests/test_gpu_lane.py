@@ -123,3 +123,39 @@ def test_lane_narrow_slabs(dev, n, shape_of):
     _with_variant(n, 5, lambda: core.solve_axis(Plan(n, co), x, axis))
     want = np.moveaxis(cr.cr_solve(co, np.moveaxis(d, axis, -1)), -1, axis)
     assert rel_err(x.cpu().numpy(), want) < 1e-12
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+@pytest.mark.parametrize("n", [64, 96, 128, 256, 512])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_lane_fp32_storage_fp32_refine(dev, n, axis, sweeps):
+    """FP32-stored fields with the FP32 solve and FP64-residual sweeps: with one
+    sweep and n a multiple of 16 the FP32 lane solve runs (every block full),
+    with two the previous FP32 tile kernels do.  Both equal the oracle's
+    derivative of the same FP32 values to the refinement tolerance (1e-10) and
+    each other to 1e-12; the input is left untouched."""
+    from ntamd import COMPACT_COEFFS, Plan, core
+    from ntamd.autotune import _set, current
+    from ntamd.workloads import compact_field
+    shape = SHAPES[axis](n)
+    f32 = compact_field(shape, seed=5 * n + axis).astype(np.float32)
+    h = 1.0 / (n - 1)
+    want = compact.derivative_banded(f32.astype(np.float64), h, axis)
+    fd = torch.tensor(f32, device=dev)
+    plan = Plan(n, COMPACT_COEFFS)
+
+    def run(v):
+        keep, _ = current(1, n, "fp32_refine")
+        _set(1, "fp32_refine", n, v)
+        try:
+            out = torch.full(shape, float("nan"), dtype=torch.float64, device=dev)
+            core.compact_derivative(plan, fd, out, h, axis, "fp32_refine", sweeps)
+            return out.cpu().numpy()
+        finally:
+            _set(1, "fp32_refine", n, keep)
+
+    got = run(5)
+    assert not np.isnan(got).any()
+    assert rel_err(got, want) < 1e-10
+    assert np.abs(got - run(2)).max() / np.abs(want).max() < 1e-12
+    assert np.array_equal(fd.cpu().numpy(), f32)
