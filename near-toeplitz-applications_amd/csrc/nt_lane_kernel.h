@@ -5,9 +5,11 @@
 // rows of all 64 lines, one line per lane, in registers: M rows for w < j and
 // M - 1 for the others (the plan's m and j; M = ceil(n / 16)), where the last
 // register row is padding that is never loaded or stored and holds zero.
-// Each row access of a wavefront is one 512-byte run inside one plane, so
-// z-lines (rows a plane apart) touch one page per load instruction, where a
-// 16-line LDS tile touches eight.  Nothing is transposed.
+// Each row access of a wavefront is one 512-byte run inside one plane (FP32
+// loads: 256 bytes), so z-lines (rows a plane apart) touch one page per load
+// instruction, where a 16-line LDS tile touches eight.  Nothing is
+// transposed.  line_kernel_lanes below is the FP64 solve; line_kernel_lanes_f32r
+// further down the FP32 solve with one FP64-residual sweep for FP32 storage.
 //
 // Per workgroup (the factors are the plan's NtLaneTab, read by scalar loads
 // from the kernel arguments):
