@@ -19,7 +19,8 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 
 // Launch-shape variants, chosen per (layout, precision, ceil(log2 n), n a power of two)
 // by the tuning table
-// (nt_tune_set; defaults measured on MI355X).  Every variant computes the same values.
+// (nt_tune_set; defaults measured on MI355X).  The variants agree to within a few
+// roundings (the lane-per-line solves round differently from cyclic reduction).
 //   CONTIG  0: one line per wavefront, wave-coalesced nontemporal loads and
 //              stores, wave-private LDS transpose (needs 16-byte aligned lines)
 //           1: one line per wavefront, each lane loads its own chunk directly
