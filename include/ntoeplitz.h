@@ -59,7 +59,7 @@ int nt_version(void);
  * per wavefront, 4 the two lines on 32-lane segments, 5 one lane per line with
  * a block-partitioned FP64 solve (49..512-point lines, FP64 or FP32 storage),
  * or at NT_PREC_F32_REFINE the FP32 block solve with one sweep of FP32-stored
- * compact derivatives (n a multiple of 16 from 64 to 512); lines variant 5
+ * compact derivatives (49..496 and 512 points); lines variant 5
  * does not take run the shapes that are the defaults without it (not for ADI,
  * whose layout 3 has variants 0-3). */
 int nt_tune_set(int layout, int precision, int64_t n, int variant);

@@ -71,12 +71,13 @@ int g_tune[4][2][13][2] = {
     // 395-412.  FP32-stored fields (FP64 solve, 2^27 points): 128 points 866 -> 328,
     // 827 -> 286; 384: 581 -> 331, 541 -> 294; 512^3: 372 -> 321, 340 -> 312
     // FP32 + refinement: 5 takes the FP32 lane solve of FP32-stored compact
-    // derivatives with one sweep (n = 16 m; 2^27-point fields, d/dz and d/dy against
-    // the FP32 tile kernels: 128 points 992 -> 376, 956 -> 358 us; 256: 645 -> 351,
-    // 616 -> 334; 512: 425 -> 367, 393 -> 349; 64: 678 -> 590, 642 -> 547) and gives
-    // everything else the shapes before it (lane_fallback in nt_launch.h)
+    // derivatives with one sweep (2^27-point fields, d/dz and d/dy against the FP32
+    // tile kernels: 128 points 992 -> 376, 956 -> 358 us; 256: 645 -> 351, 616 -> 334;
+    // 512: 425 -> 367, 393 -> 349; 64: 678 -> 590, 642 -> 547; 100: 1796 -> 888,
+    // 1782 -> 831; 300: 931 -> 357, 916 -> 341; 450: 677 -> 320, 618 -> 317) and
+    // gives everything else the shapes before it (lane_fallback in nt_launch.h)
     {{{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {5, 5}, {5, 5}, {5, 5}, {5, 5}, {2, 2}, {2, 1}, {2, 1}},
-     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {4, 5}, {5, 5}, {5, 5}, {5, 5}, {2, 2}, {2, 1}, {1, 1}}},
+     {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {5, 5}, {5, 5}, {5, 5}, {5, 5}, {2, 2}, {2, 1}, {1, 1}}},
     // ADI x half step (contiguous lines), variants as layout 0: at 2048 points the
     // wave-private transpose, which forms the stencil from coalesced reads of the
     // neighbouring lines (ADI 2048^2 step 58.7 -> 52.3 us; the compact derivative

@@ -184,8 +184,9 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes(LaneArgs arg
 }
 
 // FP32-stored compact derivative with the FP32 solve and one FP64-residual
-// refinement sweep, one lane per line, every block full (n = 16 M).  A lane
-// keeps its block's FP32 field values and FP32 solution in registers:
+// refinement sweep, one lane per line, with the blocks of line_kernel_lanes
+// (M or M - 1 rows; EVEN: all M).  A lane keeps its block's FP32 field values
+// and FP32 solution in registers (a padding row holds zero throughout):
 //   1. load the FP32 values; the block's end rows through LDS; barrier;
 //   2. x = FP32(d), d the compact right-hand side formed in FP64
 //      (compact/rhs.py:21-36, one-sided closures at the line ends);
@@ -210,7 +211,7 @@ __device__ __forceinline__ const LaneArgs32& lane32_args() {
   return *(const LaneArgs32*)(__builtin_amdgcn_kernarg_segment_ptr());
 }
 
-template <int M>
+template <int M, bool EVEN>
 __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArgs32 args_in) {
   (void)args_in;
   static_assert(M >= 4 && M <= NT_LANE_MAX_M, "rows per block");
@@ -224,12 +225,16 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArg
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   const bool lastw = (w == NW - 1);
+  const int jl = A.t.j;
+  const bool full = EVEN || w < jl;           // this block has M rows (else M - 1)
+  const bool endrow2 = lastw && !full;        // row M-2 is the line's last row
   const int64_t slab = static_cast<int64_t>(blockIdx.x) / A.groups;
   const int64_t g0 = (static_cast<int64_t>(blockIdx.x) - slab * A.groups) * 64;
   const int64_t left = A.phys - g0;   // lines in this group: 1..64 used
   const bool on = lane < left;
   const int64_t lc = on ? lane : left - 1;   // lanes past the last line re-read it, store nothing
-  const int64_t row0 = slab * A.ostride + g0 + static_cast<int64_t>(w) * M * A.estride;
+  const int first_row = w * (M - 1) + (w < jl ? w : jl);
+  const int64_t row0 = slab * A.ostride + g0 + static_cast<int64_t>(first_row) * A.estride;
   const int64_t es = A.estride;
 
   // 1. load
@@ -237,10 +242,14 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArg
   {
     const float* src = A.in + row0 + lc;
 #pragma unroll
-    for (int r = 0; r < M; ++r) f[r] = __builtin_nontemporal_load(src + r * es);
+    for (int r = 0; r < M - 1; ++r) f[r] = __builtin_nontemporal_load(src + r * es);
+    if (full)
+      f[M - 1] = __builtin_nontemporal_load(src + (M - 1) * es);
+    else
+      f[M - 1] = 0.0f;
   }
   edge[0][w][lane] = f[0];
-  edge[1][w][lane] = f[M - 1];
+  edge[1][w][lane] = full ? f[M - 1] : f[M - 2];
   __syncthreads();
   const float fleft = (w > 0) ? edge[1][w > 0 ? w - 1 : 0][lane] : 0.0f;
   const float fright = (w < NW - 1) ? edge[0][w < NW - 1 ? w + 1 : 0][lane] : 0.0f;
@@ -248,29 +257,34 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArg
   const double b0 = (w == 0) ? s2 * (-5.0 * static_cast<double>(f[0]) + 4.0 * static_cast<double>(f[1]) +
                                      static_cast<double>(f[2]))
                              : 0.0;
-  const double bn = lastw ? -s2 * (-5.0 * static_cast<double>(f[M - 1]) + 4.0 * static_cast<double>(f[M - 2]) +
-                                   static_cast<double>(f[M - 3]))
-                          : 0.0;
+  const double bn = !lastw ? 0.0
+                    : full ? -s2 * (-5.0 * static_cast<double>(f[M - 1]) + 4.0 * static_cast<double>(f[M - 2]) +
+                                    static_cast<double>(f[M - 3]))
+                           : -s2 * (-5.0 * static_cast<double>(f[M - 2]) + 4.0 * static_cast<double>(f[M - 3]) +
+                                    static_cast<double>(f[M - 4]));
+  // right-hand side of row r from its neighbours' field values, in FP64
+  auto rhs = [&](int r, float fm, float fp) -> double {
+    double d = s1 * (static_cast<double>(fp) - static_cast<double>(fm));
+    if (r == 0 && w == 0) d = b0;
+    if (lastw && (full ? r == M - 1 : r == M - 2)) d = bn;
+    if (!full && r == M - 1) d = 0.0;   // padding
+    return d;
+  };
 
   // 2. x = FP32(d)
   float x[M];
 #pragma unroll
   for (int r = 0; r < M; ++r) {
     const float fm = (r > 0) ? f[r > 0 ? r - 1 : 0] : fleft;
-    const float fp = (r < M - 1) ? f[r < M - 1 ? r + 1 : 0] : fright;
-    double d = s1 * (static_cast<double>(fp) - static_cast<double>(fm));
-    if (r == 0 && w == 0) d = b0;
-    if (r == M - 1 && lastw) d = bn;
-    x[r] = static_cast<float>(d);
+    const float fp = (r + 2 < M) ? f[r + 2 < M ? r + 1 : 0] : ((r + 2 == M && full) ? f[M - 1] : fright);
+    x[r] = static_cast<float>(rhs(r, fm, fp));
   }
 
   const float* ip = A.t.ip[w == 0 ? 0 : 1];
   const float* cp = A.t.cp[w == 0 ? 0 : 1];
-  const float* uh = A.t.uh[lastw ? 1 : 0];
-  const float* lh = A.t.lh[w == 0 ? 0 : 1];
+  const float* uh = A.t.uh[lastw ? 2 : (full ? 0 : 1)];
+  const float* lh = A.t.lh[w == 0 ? 0 : (full ? 1 : 2)];
   const float ai = A.t.ai;
-  const float al = lastw ? A.t.an : ai;
-  const float il = lastw ? A.t.ip_last : ip[M - 1];
   // the reduced system of the 64 lines: this block's first and last values
   // in, alpha (previous block's last) and beta (next block's first) out
   auto reduced = [&](float first, float last, float& alpha, float& beta) {
@@ -295,16 +309,18 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArg
     alpha = (w >= 1) ? red[w >= 1 ? 2 * w - 1 : 0][lane] : 0.0f;
     beta = (w <= NW - 2) ? red[w <= NW - 2 ? 2 * w : 0][lane] : 0.0f;
   };
-  // FP32 block sweep of v in place, then the reduced system and the correction
+  // FP32 block sweep of v in place (a padding row stays zero), then the
+  // reduced system and the correction
   auto solve32 = [&](float (&v)[M]) {
     v[0] *= ip[0];
 #pragma unroll
-    for (int r = 1; r < M - 1; ++r) v[r] = (v[r] - ai * v[r - 1]) * ip[r];
-    v[M - 1] = (v[M - 1] - al * v[M - 2]) * il;
+    for (int r = 1; r < M - 2; ++r) v[r] = (v[r] - ai * v[r - 1]) * ip[r];
+    v[M - 2] = (v[M - 2] - (endrow2 ? A.t.an : ai) * v[M - 3]) * (endrow2 ? A.t.ip_last : ip[M - 2]);
+    if (full) v[M - 1] = (v[M - 1] - (lastw ? A.t.an : ai) * v[M - 2]) * (lastw ? A.t.ip_last : ip[M - 1]);
 #pragma unroll
     for (int r = M - 2; r >= 0; --r) v[r] -= cp[r] * v[r + 1];
     float alpha, beta;
-    reduced(v[0], v[M - 1], alpha, beta);
+    reduced(v[0], full ? v[M - 1] : v[M - 2], alpha, beta);
 #pragma unroll
     for (int r = 0; r < M; ++r) v[r] += alpha * uh[r] + beta * lh[r];
   };
@@ -315,7 +331,7 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArg
   // 4. residual in FP64, over the field values (the edge rows were read before
   // the reduced system's first barrier, so they can be reused)
   edge[0][w][lane] = x[0];
-  edge[1][w][lane] = x[M - 1];
+  edge[1][w][lane] = full ? x[M - 1] : x[M - 2];
   __syncthreads();
   {
     const float xl = (w > 0) ? edge[1][w > 0 ? w - 1 : 0][lane] : 0.0f;
@@ -325,19 +341,20 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArg
 #pragma unroll
     for (int r = 0; r < M; ++r) {
       const float fm = (r > 0) ? f[r > 0 ? r - 1 : 0] : fleft;
-      const float fp = (r < M - 1) ? f[r < M - 1 ? r + 1 : 0] : fright;
-      double d = s1 * (static_cast<double>(fp) - static_cast<double>(fm));
-      if (r == 0 && w == 0) d = b0;
-      if (r == M - 1 && lastw) d = bn;
+      const float fp = (r + 2 < M) ? f[r + 2 < M ? r + 1 : 0] : ((r + 2 == M && full) ? f[M - 1] : fright);
+      const double d = rhs(r, fm, fp);
       const double xm = static_cast<double>((r > 0) ? x[r > 0 ? r - 1 : 0] : xl);
       const double xc = static_cast<double>(x[r]);
-      const double xp = static_cast<double>((r < M - 1) ? x[r < M - 1 ? r + 1 : 0] : xr);
+      const double xp = static_cast<double>((r + 2 < M) ? x[r + 2 < M ? r + 1 : 0]
+                                                        : ((r + 2 == M && full) ? x[M - 1] : xr));
       double ax = aI * xm + bI * xc + cI * xp;
       if (r == 0 && w == 0) ax = b1 * xc + c1 * xp;
-      if (r == M - 1 && lastw) ax = aN * xm + bN * xc;
+      if (lastw && (full ? r == M - 1 : r == M - 2)) ax = aN * xm + bN * xc;
+      float rr = static_cast<float>(d - ax);
+      if (!full && r == M - 1) rr = 0.0f;   // padding
       // row r-1's field value is used for the last time by row r
       if (r > 0) f[r > 0 ? r - 1 : 0] = pending;
-      pending = static_cast<float>(d - ax);
+      pending = rr;
     }
     f[M - 1] = pending;
   }
@@ -349,8 +366,10 @@ __global__ void __launch_bounds__(NT_LANE_P * 64) line_kernel_lanes_f32r(LaneArg
   if (on) {
     double* dst = A.out + row0 + lc;
 #pragma unroll
-    for (int r = 0; r < M; ++r)
+    for (int r = 0; r < M - 1; ++r)
       __builtin_nontemporal_store(static_cast<double>(x[r]) + static_cast<double>(f[r]), dst + r * es);
+    if (full)
+      __builtin_nontemporal_store(static_cast<double>(x[M - 1]) + static_cast<double>(f[M - 1]), dst + (M - 1) * es);
   }
 }
 

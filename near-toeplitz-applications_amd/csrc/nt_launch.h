@@ -49,7 +49,7 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st);
 //              block-partitioned FP64 solve (49- to 512-point lines with
 //              adjacent lines adjacent in memory, FP64 or FP32 storage, outside
 //              ADI); with FP32 + refinement, the FP32 block solve of FP32-stored
-//              compact derivatives with one sweep (n = 16 m, 64..512 points);
+//              compact derivatives with one sweep (49..496 and 512 points);
 //              other lines take lane_fallback()'s shape
 // Contiguous lines that variants 0/1 cannot take (n <= 64, unaligned) use 2.
 // launch-shape variants per tuning layout: 0 contiguous, 1 strided, 2 and 3 the
@@ -289,18 +289,30 @@ hipError_t launch_lanes(const LaneArgs& la, int m, unsigned grid, hipStream_t st
   } while (0)
 
 // FP32 + refinement lane solve of FP32-stored compact derivatives (STRIDED
-// variant 5, nt_lane_kernel.h), every block full: the instance for m rows.
+// variant 5, nt_lane_kernel.h): the instance for m rows.
 template <int M = 4>
 hipError_t launch_lanes_f32r(const LaneArgs32& la, int m, unsigned grid, hipStream_t st) {
   if constexpr (M > NT_LANE_MAX_M) {
     return hipErrorInvalidValue;
   } else {
     if (m == M) {
-      hipLaunchKernelGGL((line_kernel_lanes_f32r<M>), dim3(grid), dim3(NT_LANE_P * 64), 0, st, la);
+      if (la.t.j == NT_LANE_P) {
+        hipLaunchKernelGGL((line_kernel_lanes_f32r<M, true>), dim3(grid), dim3(NT_LANE_P * 64), 0, st, la);
+      } else {
+        // 32 rows with a padding row spill 106 VGPRs: not instantiated (lane32_ok)
+        if constexpr (M == NT_LANE_MAX_M) return hipErrorInvalidValue;
+        else hipLaunchKernelGGL((line_kernel_lanes_f32r<M, false>), dim3(grid), dim3(NT_LANE_P * 64), 0, st, la);
+      }
       return hipGetLastError();
     }
     return launch_lanes_f32r<M + 1>(la, m, grid, st);
   }
+}
+
+// The FP32 lane solve takes the plan's blocks unless they are 32 rows with
+// padding rows (497..511-point lines).
+inline bool lane32_ok(const NtLaneTab32* t) {
+  return t && (t->j == NT_LANE_P || t->m < NT_LANE_MAX_M);
 }
 
 #define NT_GOL32()                                                                         \
@@ -594,7 +606,7 @@ hipError_t launch_line(LineArgs& a, int64_t outer, hipStream_t st) {
     }
     if constexpr (std::is_same_v<TIn, float> && std::is_same_v<TOut, double> && PROD == P_COMPACT &&
                   PREC == PREC_F32R) {
-      if (variant == 5 && a.lane32 && a.pstride == 1 && a.sweeps == 1) NT_GOL32();
+      if (variant == 5 && lane32_ok(a.lane32) && a.pstride == 1 && a.sweeps == 1) NT_GOL32();
     }
     // lines the lane kernels do not take (FP64 below 49 points: 32-line tiles,
     // 33..48 points on 2^26-point fields 441-613 us against 523-744 for the

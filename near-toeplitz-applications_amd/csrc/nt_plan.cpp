@@ -288,22 +288,20 @@ bool nt_lane_build(NtLaneTab* t, int64_t n, const double coeffs[7]) {
 
 bool nt_lane_build32(NtLaneTab32* t, const NtLaneTab& d) {
   std::memset(t, 0, sizeof(*t));
-  if (d.m < 4 || d.j != NT_LANE_P) return false;
+  if (d.m < 4) return false;
   const int m = d.m;
   t->ai = static_cast<float>(d.ai);
   t->an = static_cast<float>(d.an);
   t->ip_last = static_cast<float>(d.ip_last);
-  // every block full: interior blocks use uh[0] and lh[1], the last block
-  // uh[2], the first block lh[0]
   for (int r = 0; r < m; ++r) {
     for (int k = 0; k < 2; ++k) {
       t->ip[k][r] = static_cast<float>(d.ip[k][r]);
       t->cp[k][r] = static_cast<float>(d.cp[k][r]);
     }
-    t->uh[0][r] = static_cast<float>(d.uh[0][r]);
-    t->uh[1][r] = static_cast<float>(d.uh[2][r]);
-    t->lh[0][r] = static_cast<float>(d.lh[0][r]);
-    t->lh[1][r] = static_cast<float>(d.lh[1][r]);
+    for (int k = 0; k < 3; ++k) {
+      t->uh[k][r] = static_cast<float>(d.uh[k][r]);
+      t->lh[k][r] = static_cast<float>(d.lh[k][r]);
+    }
   }
   for (int i = 0; i < NT_LANE_NR; ++i) {
     t->rsub[i] = static_cast<float>(d.rsub[i]);
@@ -311,10 +309,10 @@ bool nt_lane_build32(NtLaneTab32* t, const NtLaneTab& d) {
     t->rcp[i] = static_cast<float>(d.rcp[i]);
   }
   bool ok = std::isfinite(t->ai) && std::isfinite(t->an) && std::isfinite(t->ip_last);
-  for (int r = 0; r < m; ++r)
-    for (int k = 0; k < 2; ++k)
-      ok = ok && std::isfinite(t->ip[k][r]) && std::isfinite(t->cp[k][r]) && std::isfinite(t->uh[k][r]) &&
-           std::isfinite(t->lh[k][r]);
+  for (int r = 0; r < m; ++r) {
+    for (int k = 0; k < 2; ++k) ok = ok && std::isfinite(t->ip[k][r]) && std::isfinite(t->cp[k][r]);
+    for (int k = 0; k < 3; ++k) ok = ok && std::isfinite(t->uh[k][r]) && std::isfinite(t->lh[k][r]);
+  }
   for (int i = 0; i < NT_LANE_NR; ++i)
     ok = ok && std::isfinite(t->rsub[i]) && std::isfinite(t->rip[i]) && std::isfinite(t->rcp[i]);
   if (!ok) {
@@ -322,5 +320,6 @@ bool nt_lane_build32(NtLaneTab32* t, const NtLaneTab& d) {
     return false;
   }
   t->m = m;
+  t->j = d.j;
   return true;
 }

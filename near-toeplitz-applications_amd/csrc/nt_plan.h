@@ -90,15 +90,15 @@ struct NtLaneTab {
 };
 
 // The same factors rounded to FP32, for the FP32 + refinement lane solve of
-// FP32-stored fields: every block full (n = 16 m, 4 <= m <= 32).
+// FP32-stored fields (same m, j and array meanings as NtLaneTab).
 struct NtLaneTab32 {
-  int32_t m;                        // rows per block; 0: no FP32 lane solve for this plan
-  int32_t pad_;
-  float ai, an, ip_last, pad2_;
+  int32_t m;                        // rows of the long blocks; 0: no FP32 lane solve for this plan
+  int32_t j;                        // blocks 0..j-1 have m rows, blocks j..P-1 have m - 1
+  float ai, an, ip_last, pad_;
   float ip[2][NT_LANE_MAX_M];       // [0] first block, [1] the others
   float cp[2][NT_LANE_MAX_M];
-  float uh[2][NT_LANE_MAX_M];       // response to alpha: [0] interior blocks, [1] the last block
-  float lh[2][NT_LANE_MAX_M];       // response to beta: [0] the first block, [1] interior blocks
+  float uh[3][NT_LANE_MAX_M];       // [0] interior blocks of m rows, [1] of m - 1 rows, [2] the last block
+  float lh[3][NT_LANE_MAX_M];       // [0] the first block, [1] interior blocks of m rows, [2] of m - 1 rows
   float rsub[NT_LANE_NR], rip[NT_LANE_NR], rcp[NT_LANE_NR];
 };
 
@@ -110,7 +110,7 @@ struct nt_plan {
   double lev[NT_MAX_LEVELS][8];      // m, bF, cF, a, b, c, aL, bL per level
   NtPlanK k;
   NtLaneTab lane;                    // block-partitioned lane solve (lane.m == 0: none)
-  NtLaneTab32 lane32;                // its FP32 form when every block is full (lane32.m == 0: none)
+  NtLaneTab32 lane32;                // its FP32 form (lane32.m == 0: none)
 };
 
 // Fill `p` for an n x n near-Toeplitz matrix.  Returns 0, or a negative code
@@ -123,6 +123,6 @@ int nt_plan_build(nt_plan* p, int64_t n, const double coeffs[7], char* err, int 
 // Thomas solve by more than 1e-11 relative.
 bool nt_lane_build(NtLaneTab* t, int64_t n, const double coeffs[7]);
 
-// t = d rounded to FP32 when d has every block full (d.j == 16); false and
-// t->m = 0 otherwise.
+// t = d rounded to FP32; false and t->m = 0 when d holds no tables or a
+// factor overflows FP32.
 bool nt_lane_build32(NtLaneTab32* t, const NtLaneTab& d);

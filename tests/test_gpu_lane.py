@@ -126,14 +126,14 @@ def test_lane_narrow_slabs(dev, n, shape_of):
 
 
 @pytest.mark.parametrize("sweeps", [1, 2])
-@pytest.mark.parametrize("n", [64, 96, 128, 256, 512])
+@pytest.mark.parametrize("n", [52, 64, 96, 100, 128, 200, 256, 300, 480, 497, 512])
 @pytest.mark.parametrize("axis", [0, 1])
 def test_lane_fp32_storage_fp32_refine(dev, n, axis, sweeps):
     """FP32-stored fields with the FP32 solve and FP64-residual sweeps: with one
-    sweep and n a multiple of 16 the FP32 lane solve runs (every block full),
-    with two the previous FP32 tile kernels do.  Both equal the oracle's
-    derivative of the same FP32 values to the refinement tolerance (1e-10) and
-    each other to 1e-12; the input is left untouched."""
+    sweep the FP32 lane solve runs (full and padded blocks; 497 points, 32 rows
+    with padding, and two sweeps take the previous FP32 tile kernels).  Both
+    equal the oracle's derivative of the same FP32 values to the refinement
+    tolerance (1e-10) and each other to 1e-12; the input is left untouched."""
     from ntamd import COMPACT_COEFFS, Plan, core
     from ntamd.autotune import _set, current
     from ntamd.workloads import compact_field
